@@ -1,0 +1,140 @@
+/*
+ * mafem.h -- C ABI of the MI355X (gfx950) implementation of MembraneAleFem.jl's
+ * per-time-step hot path: element residual/tangent assembly and the Newton
+ * sparse linear solve.
+ *
+ * All arrays crossing this boundary use the reference's own layouts: Julia
+ * column-major matrices of Int64 (1-based indices) and Float64, so a Julia
+ * caller passes `mesh.IX`, `mesh.ID`, `xms`, `cps`, ... by pointer with no
+ * conversion.  Plain pointers and sizes only.
+ *
+ * Return codes: MAFEM_OK (0), MAFEM_NOT_CONVERGED (1: Newton did not reach
+ * the tolerance, the reference raises AssertionError), negative on error
+ * (invalid argument, HIP runtime error, device memory exhausted, unsupported
+ * configuration).  mafem_last_error() describes the most recent failure.
+ *
+ * Threading: one context per host thread at a time; calls on a context are
+ * blocking and not re-entrant.
+ */
+#ifndef MAFEM_H
+#define MAFEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MAFEM_OK 0
+#define MAFEM_NOT_CONVERGED 1
+#define MAFEM_EINVAL (-1)
+#define MAFEM_EHIP (-2)
+#define MAFEM_ENOMEM (-3)
+#define MAFEM_EUNSUPPORTED (-4)
+
+typedef struct mafem_ctx mafem_ctx;
+
+/* Mesh tables as the reference's `Mesh` struct holds them
+ * (src/input/Mesh.jl:48-77).  Boundary arrays are indexed by the Boundary
+ * enum minus one: BOTTOM, RIGHT, TOP, LEFT (src/input/Enums.jl:89-94). */
+typedef struct {
+  int64_t num1el, num2el, numel;   /* Mesh.jl:50-52 */
+  int64_t num1np, num2np, numnp;   /* Mesh.jl:53-55 */
+  int64_t ndf, nmdf;               /* Mesh.jl:71,74 */
+  const int64_t *IX;               /* NEN x numel, 1-based node ids (Mesh.jl:56) */
+  const int64_t *ID;               /* ndf x numnp, 1-based equation ids, 0 = prescribed (Mesh.jl:72) */
+  /* mesh.dofs as column indices of cps (get_v_order/get_m_order/get_λ_order/
+   * get_p_order, Mesh.jl:477-521); 0 = unknown absent */
+  int32_t v_order[3], m_order[3], lam_order, p_order;
+  /* area basis tables (mesh.area_gp_fns, GpBasisFn.jl:300-356): ufns is the
+   * Matrix{GpBasisFnsζα}(n_uel, 9) in Julia memory order, each record 55
+   * Float64 = w, N[9], ∂Nα[9x2], ∂∂Nαβ[9x3] (GpBasisFn.jl:96-112) */
+  int64_t area_n_uel;
+  const int64_t *area_uel_ids;     /* numel, 1-based */
+  const double *area_ufns;         /* 55 * n_uel * 9 */
+  /* boundary elements and basis tables (mesh.bdry_elems, mesh.bdry_gp_fns,
+   * Mesh.jl:57,66; GpBasisFn.jl:231-277): ufns is Matrix(n_uel, 3) */
+  int64_t bdry_n_el[4];
+  const int64_t *bdry_elems[4];    /* 1-based area element ids */
+  int64_t bdry_n_uel[4];
+  const int64_t *bdry_uel_ids[4];  /* bdry_n_el[b] entries, 1-based */
+  const double *bdry_ufns[4];      /* 55 * n_uel * 3 */
+  /* mesh.inh_neu_bcs (Mesh.jl:77): (Boundary, Neumann, value) triples */
+  int64_t n_neu;
+  const int32_t *neu_bdry;         /* Boundary enum, 1..4 */
+  const int32_t *neu_type;         /* Neumann enum: SHEAR 1, STRETCH 2, MOMENT 3 */
+  const double *neu_val;
+} mafem_mesh_desc;
+
+/* The hot-path subset of `Params` (src/input/Params.jl:36-55) plus the
+ * `bend_tm` keyword argument used by MOMENT edges (FiniteElement.jl:379). */
+typedef struct {
+  int32_t motion;                  /* Motion enum: STATIC 1, EUL 2, LAG 3, ALEV 4, ALEVB 5 */
+  int32_t scenario;                /* Scenario enum: F_CAVI 1 .. F_BEND 5 */
+  double kb, kg, zv, pn, adb, am;  /* kb, kg, ζv, pn, αdb, αm */
+  double ek, enr;                  /* εk (unused: the tangent is exact), εnr */
+  double bend_tm;
+} mafem_params;
+
+/* Per-phase device timings (microseconds) of the last mafem_time_step or
+ * mafem_assemble call, measured with HIP events on the context's stream. */
+typedef struct {
+  double assemble_us, factor_us, solve_us, update_us, total_us;
+  int32_t newton_iters, refine_sweeps;
+  double last_relres;              /* |K du + r| / |r| of the last solve */
+} mafem_stats;
+
+/* Builds the device context: uploads the tables, builds the block-sparse
+ * pattern, the deterministic gather lists and the band layout once per mesh.
+ * Replaces the per-call set-up of calc_r_K (FiniteElement.jl:85-95). */
+int mafem_create(mafem_ctx **ctx, const mafem_mesh_desc *mesh, const mafem_params *p);
+void mafem_destroy(mafem_ctx *ctx);
+/* Text of the last error on ctx (or of the last failed mafem_create when ctx
+ * is NULL). */
+const char *mafem_last_error(const mafem_ctx *ctx);
+
+/* Device-resident state: xms is numnp x 3, cps numnp x ndf, column-major,
+ * exactly the reference's `xms::Matrix{Float64}`, `cps::Matrix{Float64}`. */
+int mafem_set_state(mafem_ctx *ctx, const double *xms, const double *cps);
+int mafem_get_state(mafem_ctx *ctx, double *xms, double *cps);
+
+/* Sets prescribed values in the device state: cps[node, dof] = val for n
+ * (1-based node, 1-based dof column) triples, as prepare_input applies
+ * mesh.inh_dir_bcs (src/Input.jl:103-108). */
+int mafem_set_prescribed(mafem_ctx *ctx, int64_t n, const int64_t *node,
+                         const int32_t *dof, const double *val);
+
+/* update_xms!(motion, xms, cps, Δt, dofs): the predictor x += Δt u_motion
+ * applied before each step (FiniteElement.jl:408-423, Analysis.jl:70). */
+int mafem_predict(mafem_ctx *ctx, double dt);
+
+/* Number of stored entries of the exported tangent (the structural superset
+ * of the reference's value-dependent pattern: every active DOF pair of two
+ * nodes sharing an element). */
+int mafem_nnz(mafem_ctx *ctx, int64_t *nnz);
+
+/* calc_r_K (FiniteElement.jl:75-200) on the device state: writes r_gl
+ * (nmdf) and K_gl as 1-based CSC (colptr nmdf+1, rowval/nzval nnz).  Any
+ * output pointer may be NULL to skip it. */
+int mafem_assemble(mafem_ctx *ctx, double time, double dt, double *r,
+                   int64_t *colptr, int64_t *rowval, double *nzval);
+
+/* Solves K du = rhs with the tangent of the last mafem_assemble (direct band
+ * LU on the device plus FP64 refinement); rhs, du have nmdf entries.
+ * Replaces `K_gl \ r_gl` (FiniteElement.jl:38). */
+int mafem_solve(mafem_ctx *ctx, const double *rhs, double *du, double *relres);
+
+/* time_step! (FiniteElement.jl:11-63): Newton on the device state, at most
+ * max_iter iterations (the reference allows 14); eps_hist receives
+ * |Δu|_2 / nmdf per iteration (may be NULL), iters the iteration count.
+ * Returns MAFEM_NOT_CONVERGED if the last eps >= εnr. */
+int mafem_time_step(mafem_ctx *ctx, double time, double dt, int32_t max_iter,
+                    int32_t *iters, double *eps_hist);
+
+int mafem_get_stats(const mafem_ctx *ctx, mafem_stats *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MAFEM_H */

@@ -1,0 +1,628 @@
+// C ABI of libmafem (declared in include/mafem.h).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+
+using namespace mafem;
+
+struct mafem_ctx {
+  Ctx c;
+};
+
+static thread_local std::string g_create_err;
+
+namespace {
+
+struct ArgError {
+  std::string msg;
+  int code;
+};
+
+template <class T>
+T *dalloc(Ctx &c, int64_t n) {
+  void *p = nullptr;
+  if (n <= 0) n = 1;
+  hipError_t e = hipMalloc(&p, sizeof(T) * (size_t)n);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    throw ArgError{"device allocation of " + std::to_string(sizeof(T) * (size_t)n) +
+                       " bytes failed",
+                   MAFEM_ENOMEM};
+  }
+  return (T *)p;
+}
+
+template <class T>
+T *dupload(Ctx &c, const std::vector<T> &v) {
+  T *p = dalloc<T>(c, (int64_t)v.size());
+  if (!v.empty())
+    MAFEM_HIP(hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+  return p;
+}
+
+void free_all(Ctx &c) {
+  void *ptrs[] = {c.d_xms,  c.d_cps,    c.d_ix,      c.d_uel,     c.d_active, c.d_ufns,
+                  c.d_dbt,  c.d_bufns,  c.d_edges,   c.d_kel,     c.d_rel,    c.d_rowptr,
+                  c.d_colidx, c.d_brow, c.d_bsr,     c.d_gptr,    c.d_gsrc,   c.d_rptr,
+                  c.d_rsrc, c.d_r,      c.d_x,       c.d_w,       c.d_t,      c.d_b,
+                  c.d_red,  c.d_csc_src, c.d_csc_val, c.d_eq,     c.d_band,   c.d_linv,
+                  c.d_uinv, c.d_piv};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (c.h_red) (void)hipHostFree(c.h_red);
+  for (auto &e : c.ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c.stream) (void)hipStreamDestroy(c.stream);
+}
+
+int fail(Ctx &c, const std::string &m, int code) {
+  c.err = m;
+  return code;
+}
+
+int from_hip(Ctx &c, const HipError &h) {
+  c.err = std::string("HIP error ") + hipGetErrorString(h.code) + " in " + h.what + " (" + h.file +
+          ":" + std::to_string(h.line) + ")";
+  return MAFEM_EHIP;
+}
+
+// The supported DOF layouts are those of get_dofs (src/input/Bc.jl:414-431).
+bool layout_ok(const mafem_mesh_desc *m, int motion, int &ndf) {
+  auto eq3 = [](const int32_t *a, int x, int y, int z) { return a[0] == x && a[1] == y && a[2] == z; };
+  if (!eq3(m->v_order, 1, 2, 3)) return false;
+  if (motion == LAG) {
+    ndf = 4;
+    return eq3(m->m_order, 0, 0, 0) && m->lam_order == 4 && m->p_order == 0;
+  }
+  if (motion == EUL) {
+    ndf = 7;
+    return eq3(m->m_order, 4, 5, 6) && m->lam_order == 7 && m->p_order == 0;
+  }
+  if (motion == ALEV || motion == ALEVB) {
+    ndf = 8;
+    return eq3(m->m_order, 4, 5, 6) && m->lam_order == 7 && m->p_order == 8;
+  }
+  return false;
+}
+
+void build(Ctx &c, const mafem_mesh_desc *m, const mafem_params *p) {
+  if (!m || !p) throw ArgError{"null mesh or params", MAFEM_EINVAL};
+  if (m->numel != m->num1el * m->num2el || m->numnp != m->num1np * m->num2np ||
+      m->num1np != m->num1el + 2 || m->num2np != m->num2el + 2 || m->numel <= 0)
+    throw ArgError{"inconsistent flat-patch mesh sizes", MAFEM_EINVAL};
+  int ndf = 0;
+  if (!layout_ok(m, p->motion, ndf))
+    throw ArgError{"unsupported motion / DOF layout (supported: LAG, EUL, ALEV, ALEVB with "
+                   "the get_dofs ordering)",
+                   MAFEM_EUNSUPPORTED};
+  if (m->ndf != ndf) throw ArgError{"ndf does not match the DOF layout", MAFEM_EINVAL};
+  if (!m->IX || !m->ID || !m->area_uel_ids || !m->area_ufns || m->area_n_uel <= 0)
+    throw ArgError{"missing mesh tables", MAFEM_EINVAL};
+  for (int64_t k = 0; k < m->n_neu; ++k) {
+    const int b = m->neu_bdry[k], t = m->neu_type[k];
+    if (b < 1 || b > 4 || t < 1 || t > 3) throw ArgError{"bad Neumann entry", MAFEM_EINVAL};
+    if (t == MOMENT && p->scenario != F_BEND)
+      throw ArgError{"Neumann boundary condition not implemented", MAFEM_EUNSUPPORTED};
+    if (m->bdry_n_el[b - 1] > 0 && (!m->bdry_elems[b - 1] || !m->bdry_uel_ids[b - 1] ||
+                                    !m->bdry_ufns[b - 1] || m->bdry_n_uel[b - 1] <= 0))
+      throw ArgError{"missing boundary tables", MAFEM_EINVAL};
+  }
+  if (p->zv == 0.0) throw ArgError{"zv must be nonzero", MAFEM_EINVAL};
+
+  c.num1el = m->num1el; c.num2el = m->num2el; c.numel = m->numel;
+  c.num1np = m->num1np; c.num2np = m->num2np; c.numnp = m->numnp;
+  c.ndf = ndf; c.ne = 9 * ndf; c.motion = p->motion; c.enr = p->enr;
+  c.phys = PhysParams{p->kb, p->kg, p->zv, p->pn, p->adb, p->am, p->bend_tm, p->scenario};
+  for (int i = 0; i < 3; ++i) { c.v_order[i] = m->v_order[i]; c.m_order[i] = m->m_order[i]; }
+  c.lam_order = m->lam_order; c.p_order = m->p_order;
+  for (int i = 0; i < 3; ++i) c.mm_order[i] = (p->motion == LAG) ? m->v_order[i] : m->m_order[i];
+  const int64_t numel = c.numel, numnp = c.numnp, ne = c.ne;
+
+  // IX, ID
+  std::vector<int32_t> ix(9 * numel);
+  for (int64_t k = 0; k < 9 * numel; ++k) {
+    const int64_t v = m->IX[k];
+    if (v < 1 || v > numnp) throw ArgError{"IX entry out of range", MAFEM_EINVAL};
+    ix[k] = (int32_t)(v - 1);
+  }
+  std::vector<uint8_t> active(ndf * numnp);
+  int64_t nmdf = 0;
+  std::vector<int64_t> eq_ext;  // extended index of each equation
+  for (int64_t k = 0; k < ndf * numnp; ++k) {
+    active[k] = m->ID[k] != 0;
+    nmdf = std::max<int64_t>(nmdf, m->ID[k]);
+  }
+  if (nmdf != m->nmdf) throw ArgError{"nmdf does not match max(ID)", MAFEM_EINVAL};
+  c.nmdf = nmdf;
+  eq_ext.assign(nmdf, -1);
+  for (int64_t k = 0; k < ndf * numnp; ++k)
+    if (m->ID[k] > 0) eq_ext[m->ID[k] - 1] = k;  // ID is ndf x numnp: k = d + ndf p
+  for (auto v : eq_ext)
+    if (v < 0) throw ArgError{"ID is not a numbering of 1..nmdf", MAFEM_EINVAL};
+
+  // area basis: Julia Matrix{GpBasisFnsζα}(n_uel, 9) -> [uel][gp][RECP]
+  const int64_t nu = m->area_n_uel;
+  std::vector<int32_t> uel(numel);
+  for (int64_t e = 0; e < numel; ++e) {
+    const int64_t u = m->area_uel_ids[e];
+    if (u < 1 || u > nu) throw ArgError{"area uel id out of range", MAFEM_EINVAL};
+    uel[e] = (int32_t)(u - 1);
+  }
+  std::vector<double> ufns(nu * 9 * RECP, 0.0), dbt(nu * 81);
+  const double xi[3] = {-std::sqrt(3.0 / 5.0), 0.0, std::sqrt(3.0 / 5.0)};
+  for (int64_t u = 0; u < nu; ++u) {
+    double G[3][9] = {{0}}, H[3][3] = {{0}};
+    for (int g = 0; g < 9; ++g) {
+      const double *src = m->area_ufns + REC * (u + nu * g);
+      std::memcpy(&ufns[(u * 9 + g) * RECP], src, sizeof(double) * REC);
+      // Dohrmann--Bochev matrices with N_DB = (ξ1, ξ2, 1) at the raw
+      // abscissae (FiniteElement.jl:315-320)
+      const double ndb[3] = {xi[g % 3], xi[g / 3], 1.0};
+      const double w = src[0];
+      for (int i = 0; i < 3; ++i) {
+        for (int a = 0; a < 9; ++a) G[i][a] += ndb[i] * src[1 + a] * w;
+        for (int j = 0; j < 3; ++j) H[i][j] += ndb[i] * ndb[j] * w;
+      }
+    }
+    // H^-1 by cofactors, then T = G^T H^-1 G (FiniteElement.jl:323)
+    const double c00 = H[1][1] * H[2][2] - H[1][2] * H[2][1];
+    const double c01 = H[1][2] * H[2][0] - H[1][0] * H[2][2];
+    const double c02 = H[1][0] * H[2][1] - H[1][1] * H[2][0];
+    const double det = H[0][0] * c00 + H[0][1] * c01 + H[0][2] * c02;
+    double Hi[3][3];
+    Hi[0][0] = c00 / det; Hi[1][0] = c01 / det; Hi[2][0] = c02 / det;
+    Hi[0][1] = (H[0][2] * H[2][1] - H[0][1] * H[2][2]) / det;
+    Hi[1][1] = (H[0][0] * H[2][2] - H[0][2] * H[2][0]) / det;
+    Hi[2][1] = (H[0][1] * H[2][0] - H[0][0] * H[2][1]) / det;
+    Hi[0][2] = (H[0][1] * H[1][2] - H[0][2] * H[1][1]) / det;
+    Hi[1][2] = (H[0][2] * H[1][0] - H[0][0] * H[1][2]) / det;
+    Hi[2][2] = (H[0][0] * H[1][1] - H[0][1] * H[1][0]) / det;
+    for (int a = 0; a < 9; ++a)
+      for (int b = 0; b < 9; ++b) {
+        double s = 0;
+        for (int i = 0; i < 3; ++i) {
+          double t = 0;
+          for (int j = 0; j < 3; ++j) t += Hi[i][j] * G[j][b];
+          s += G[i][a] * t;
+        }
+        dbt[u * 81 + a * 9 + b] = s;
+      }
+  }
+
+  // boundary records and edge work items, in the order of mesh.inh_neu_bcs
+  std::vector<double> bufns;
+  int64_t base[4];
+  for (int b = 0; b < 4; ++b) {
+    base[b] = (int64_t)bufns.size() / RECP;
+    const int64_t nub = m->bdry_n_uel[b];
+    for (int64_t u = 0; u < nub; ++u)
+      for (int g = 0; g < 3; ++g) {
+        const double *src = m->bdry_ufns[b] + REC * (u + nub * g);
+        const size_t off = bufns.size();
+        bufns.resize(off + RECP, 0.0);
+        std::memcpy(&bufns[off], src, sizeof(double) * REC);
+      }
+  }
+  std::vector<EdgeItem> edges;
+  for (int64_t k = 0; k < m->n_neu; ++k) {
+    const int b = m->neu_bdry[k] - 1;
+    for (int64_t j = 0; j < m->bdry_n_el[b]; ++j) {
+      const int64_t el = m->bdry_elems[b][j], u = m->bdry_uel_ids[b][j];
+      if (el < 1 || el > numel || u < 1 || u > m->bdry_n_uel[b])
+        throw ArgError{"boundary table entry out of range", MAFEM_EINVAL};
+      edges.push_back(EdgeItem{(int32_t)(el - 1), (int32_t)(base[b] + (u - 1) * 3), b + 1,
+                               m->neu_type[k], m->neu_val[k]});
+    }
+  }
+  c.n_edges = (int64_t)edges.size();
+
+  // node-block pattern: nodes sharing an element (|Δi1| <= 2, |Δi2| <= 2)
+  const int64_t n1 = c.num1np, n2 = c.num2np;
+  std::vector<int32_t> rowptr(numnp + 1), colidx, brow;
+  colidx.reserve(numnp * 25);
+  for (int64_t p = 0; p < numnp; ++p) {
+    rowptr[p] = (int32_t)colidx.size();
+    const int64_t i1 = p % n1, i2 = p / n1;
+    for (int64_t j2 = std::max<int64_t>(0, i2 - 2); j2 <= std::min(n2 - 1, i2 + 2); ++j2)
+      for (int64_t j1 = std::max<int64_t>(0, i1 - 2); j1 <= std::min(n1 - 1, i1 + 2); ++j1) {
+        colidx.push_back((int32_t)(j1 + n1 * j2));
+        brow.push_back((int32_t)p);
+      }
+  }
+  rowptr[numnp] = (int32_t)colidx.size();
+  c.nnzb = (int64_t)colidx.size();
+  auto blk_of = [&](int64_t p, int64_t q) -> int64_t {
+    const int32_t *b = colidx.data() + rowptr[p], *e = colidx.data() + rowptr[p + 1];
+    const int32_t *it = std::lower_bound(b, e, (int32_t)q);
+    return rowptr[p] + (it - b);
+  };
+
+  // deterministic gather lists (counting sort by destination; sources in
+  // element order, edges after area elements)
+  const int64_t ntot = numel + c.n_edges;
+  auto elem_nodes = [&](int64_t k, int a) -> int64_t {
+    const int64_t e = k < numel ? k : edges[k - numel].el;
+    return ix[e * 9 + a];
+  };
+  std::vector<int64_t> gptr(c.nnzb + 1, 0), rptr(numnp + 1, 0);
+  for (int64_t k = 0; k < ntot; ++k)
+    for (int a = 0; a < 9; ++a) {
+      const int64_t pa = elem_nodes(k, a);
+      rptr[pa + 1]++;
+      for (int b = 0; b < 9; ++b) gptr[blk_of(pa, elem_nodes(k, b)) + 1]++;
+    }
+  for (int64_t i = 0; i < c.nnzb; ++i) gptr[i + 1] += gptr[i];
+  for (int64_t i = 0; i < numnp; ++i) rptr[i + 1] += rptr[i];
+  std::vector<int64_t> gsrc(gptr[c.nnzb]), rsrc(rptr[numnp]);
+  {
+    std::vector<int64_t> gfill(gptr.begin(), gptr.end() - 1), rfill(rptr.begin(), rptr.end() - 1);
+    for (int64_t k = 0; k < ntot; ++k)
+      for (int a = 0; a < 9; ++a) {
+        const int64_t pa = elem_nodes(k, a);
+        rsrc[rfill[pa]++] = k * ne + a * ndf;
+        for (int b = 0; b < 9; ++b)
+          gsrc[gfill[blk_of(pa, elem_nodes(k, b))]++] = k * ne * ne + (int64_t)a * ndf * ne + b * ndf;
+      }
+  }
+
+  // CSC export over active DOFs, columns in equation order
+  c.h_colptr.assign(nmdf + 1, 1);
+  c.h_rowval.clear();
+  std::vector<int64_t> csc_src;
+  for (int64_t j = 0; j < nmdf; ++j) {
+    c.h_colptr[j] = (int64_t)c.h_rowval.size() + 1;
+    const int64_t kx = eq_ext[j], q = kx / ndf, cc = kx % ndf;
+    const int64_t q1 = q % n1, q2 = q / n1;
+    for (int64_t p2 = std::max<int64_t>(0, q2 - 2); p2 <= std::min(n2 - 1, q2 + 2); ++p2)
+      for (int64_t p1 = std::max<int64_t>(0, q1 - 2); p1 <= std::min(n1 - 1, q1 + 2); ++p1) {
+        const int64_t pp = p1 + n1 * p2;
+        const int64_t bk = blk_of(pp, q);
+        for (int r = 0; r < ndf; ++r) {
+          const int64_t id = m->ID[r + ndf * pp];
+          if (id == 0) continue;
+          c.h_rowval.push_back(id);
+          csc_src.push_back(bk * ndf * ndf + r * ndf + cc);
+        }
+      }
+  }
+  c.h_colptr[nmdf] = (int64_t)c.h_rowval.size() + 1;
+  c.nnz = (int64_t)c.h_rowval.size();
+
+  // band layout
+  c.nx = numnp * ndf;
+  c.nt = (c.nx + TILE - 1) / TILE;
+  const int64_t kl = (2 * n1 + 3) * ndf - 1;
+  c.tb = (int)((kl + TILE - 1) / TILE);
+  if (c.tb > c.nt - 1) c.tb = (int)std::max<int64_t>(1, c.nt - 1);
+  c.band_bytes = c.nt * (2 * c.tb + 1) * (int64_t)TILE * TILE * (int64_t)sizeof(double);
+
+  // device allocations and uploads
+  MAFEM_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  for (auto &e : c.ev) MAFEM_HIP(hipEventCreate(&e));
+  c.d_xms = dalloc<double>(c, numnp * 3);
+  c.d_cps = dalloc<double>(c, numnp * ndf);
+  MAFEM_HIP(hipMemset(c.d_xms, 0, sizeof(double) * numnp * 3));
+  MAFEM_HIP(hipMemset(c.d_cps, 0, sizeof(double) * numnp * ndf));
+  c.d_ix = dupload(c, ix);
+  c.d_uel = dupload(c, uel);
+  c.d_active = dupload(c, active);
+  c.d_ufns = dupload(c, ufns);
+  c.d_dbt = dupload(c, dbt);
+  c.d_bufns = dupload(c, bufns);
+  c.d_edges = dupload(c, edges);
+  c.d_kel = dalloc<double>(c, ntot * ne * ne);
+  c.d_rel = dalloc<double>(c, ntot * ne);
+  c.d_rowptr = dupload(c, rowptr);
+  c.d_colidx = dupload(c, colidx);
+  c.d_brow = dupload(c, brow);
+  c.d_bsr = dalloc<double>(c, c.nnzb * ndf * ndf);
+  c.d_gptr = dupload(c, gptr);
+  c.d_gsrc = dupload(c, gsrc);
+  c.d_rptr = dupload(c, rptr);
+  c.d_rsrc = dupload(c, rsrc);
+  const int64_t nv = c.nt * TILE;
+  c.d_r = dalloc<double>(c, nv);
+  c.d_x = dalloc<double>(c, nv);
+  c.d_w = dalloc<double>(c, nv);
+  c.d_t = dalloc<double>(c, nv);
+  c.d_b = dalloc<double>(c, nv);
+  for (double *v : {c.d_r, c.d_x, c.d_w, c.d_t, c.d_b})
+    MAFEM_HIP(hipMemset(v, 0, sizeof(double) * nv));
+  c.d_red = dalloc<double>(c, 512);
+  MAFEM_HIP(hipHostMalloc((void **)&c.h_red, sizeof(double) * 16, hipHostMallocDefault));
+  c.d_csc_src = dupload(c, csc_src);
+  c.d_csc_val = dalloc<double>(c, c.nnz);
+  c.d_eq = dupload(c, eq_ext);
+  c.d_band = dalloc<double>(c, c.band_bytes / (int64_t)sizeof(double));
+  c.d_linv = dalloc<double>(c, c.nt * TILE * TILE);
+  c.d_uinv = dalloc<double>(c, c.nt * TILE * TILE);
+  c.d_piv = dalloc<int32_t>(c, c.nt * TILE);
+}
+
+__global__ void k_gather_idx(int64_t n, const int64_t *__restrict__ idx,
+                             const double *__restrict__ src, double *__restrict__ dst) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t < n) dst[t] = src[idx[t]];
+}
+
+__global__ void k_scatter_idx(int64_t n, const int64_t *__restrict__ idx,
+                              const double *__restrict__ src, double *__restrict__ dst) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t < n) dst[idx[t]] = src[t];
+}
+
+inline unsigned nb256(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+float elapsed(Ctx &c, int a, int b) {
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, c.ev[a], c.ev[b]);
+  return ms;
+}
+
+// Solves K x = b (extended vectors) with the factored band and FP64
+// iterative refinement against the block-sparse K; returns |b - K x| / |b|.
+double solve_refined(Ctx &c, const double *d_b, double *d_x, int max_sweeps, double tol) {
+  const double bn = vec_norm2_masked(c, d_b);
+  band_solve(c, d_b, d_x);
+  double rel = 0.0;
+  int sweeps = 0;
+  if (bn == 0.0) {
+    c.stats.last_relres = 0.0;
+    c.stats.refine_sweeps = 0;
+    return 0.0;
+  }
+  for (;;) {
+    bsr_spmv(c, d_x, c.d_w);
+    vec_axpby(c, c.nx, 1.0, d_b, -1.0, c.d_w);  // w = b - K x
+    rel = vec_norm2_masked(c, c.d_w) / bn;
+    if (rel < tol || sweeps >= max_sweeps) break;
+    band_solve(c, c.d_w, c.d_t);
+    vec_axpby(c, c.nx, 1.0, c.d_t, 1.0, d_x);
+    ++sweeps;
+  }
+  c.stats.last_relres = rel;
+  c.stats.refine_sweeps = sweeps;
+  return rel;
+}
+
+}  // namespace
+
+#define ABI_GUARD(ctx, body)                                   \
+  do {                                                         \
+    if (!(ctx)) return MAFEM_EINVAL;                           \
+    Ctx &c = (ctx)->c;                                         \
+    try {                                                      \
+      body                                                     \
+    } catch (const HipError &h) {                              \
+      return from_hip(c, h);                                   \
+    } catch (const ArgError &a) {                              \
+      return fail(c, a.msg, a.code);                           \
+    } catch (const std::bad_alloc &) {                         \
+      return fail(c, "host allocation failed", MAFEM_ENOMEM);  \
+    }                                                          \
+  } while (0)
+
+extern "C" {
+
+int mafem_create(mafem_ctx **out, const mafem_mesh_desc *mesh, const mafem_params *p) {
+  if (!out) return MAFEM_EINVAL;
+  *out = nullptr;
+  mafem_ctx *ctx = new (std::nothrow) mafem_ctx();
+  if (!ctx) {
+    g_create_err = "host allocation failed";
+    return MAFEM_ENOMEM;
+  }
+  int rc = MAFEM_OK;
+  try {
+    int ndev = 0;
+    MAFEM_HIP(hipGetDeviceCount(&ndev));
+    if (ndev < 1) throw ArgError{"no HIP device", MAFEM_EHIP};
+    MAFEM_HIP(hipGetDevice(&ctx->c.dev));
+    build(ctx->c, mesh, p);
+  } catch (const HipError &h) {
+    rc = from_hip(ctx->c, h);
+  } catch (const ArgError &a) {
+    rc = fail(ctx->c, a.msg, a.code);
+  } catch (const std::bad_alloc &) {
+    rc = fail(ctx->c, "host allocation failed", MAFEM_ENOMEM);
+  }
+  if (rc != MAFEM_OK) {
+    g_create_err = ctx->c.err;
+    free_all(ctx->c);
+    delete ctx;
+    return rc;
+  }
+  *out = ctx;
+  return MAFEM_OK;
+}
+
+void mafem_destroy(mafem_ctx *ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->c.dev);
+  if (ctx->c.stream) (void)hipStreamSynchronize(ctx->c.stream);
+  free_all(ctx->c);
+  delete ctx;
+}
+
+const char *mafem_last_error(const mafem_ctx *ctx) {
+  return ctx ? ctx->c.err.c_str() : g_create_err.c_str();
+}
+
+int mafem_set_state(mafem_ctx *ctx, const double *xms, const double *cps) {
+  ABI_GUARD(ctx, {
+    if (!xms || !cps) return fail(c, "null state", MAFEM_EINVAL);
+    MAFEM_HIP(hipSetDevice(c.dev));
+    MAFEM_HIP(hipMemcpyAsync(c.d_xms, xms, sizeof(double) * c.numnp * 3, hipMemcpyHostToDevice, c.stream));
+    MAFEM_HIP(hipMemcpyAsync(c.d_cps, cps, sizeof(double) * c.numnp * c.ndf, hipMemcpyHostToDevice, c.stream));
+    MAFEM_HIP(hipStreamSynchronize(c.stream));
+    return MAFEM_OK;
+  });
+}
+
+int mafem_get_state(mafem_ctx *ctx, double *xms, double *cps) {
+  ABI_GUARD(ctx, {
+    MAFEM_HIP(hipSetDevice(c.dev));
+    if (xms)
+      MAFEM_HIP(hipMemcpyAsync(xms, c.d_xms, sizeof(double) * c.numnp * 3, hipMemcpyDeviceToHost, c.stream));
+    if (cps)
+      MAFEM_HIP(hipMemcpyAsync(cps, c.d_cps, sizeof(double) * c.numnp * c.ndf, hipMemcpyDeviceToHost, c.stream));
+    MAFEM_HIP(hipStreamSynchronize(c.stream));
+    return MAFEM_OK;
+  });
+}
+
+int mafem_set_prescribed(mafem_ctx *ctx, int64_t n, const int64_t *node, const int32_t *dof,
+                         const double *val) {
+  ABI_GUARD(ctx, {
+    if (n < 0 || (n > 0 && (!node || !dof || !val))) return fail(c, "bad prescribed list", MAFEM_EINVAL);
+    MAFEM_HIP(hipSetDevice(c.dev));
+    std::vector<int64_t> idx(n);
+    for (int64_t k = 0; k < n; ++k) {
+      if (node[k] < 1 || node[k] > c.numnp || dof[k] < 1 || dof[k] > c.ndf)
+        return fail(c, "prescribed entry out of range", MAFEM_EINVAL);
+      idx[k] = (node[k] - 1) + c.numnp * (dof[k] - 1);
+    }
+    if (n == 0) return MAFEM_OK;
+    int64_t *d_idx = dupload(c, idx);
+    double *d_val = dalloc<double>(c, n);
+    MAFEM_HIP(hipMemcpy(d_val, val, sizeof(double) * n, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_scatter_idx, dim3(nb256(n)), dim3(256), 0, c.stream, n, d_idx, d_val, c.d_cps);
+    MAFEM_HIP(hipGetLastError());
+    MAFEM_HIP(hipStreamSynchronize(c.stream));
+    (void)hipFree(d_idx);
+    (void)hipFree(d_val);
+    return MAFEM_OK;
+  });
+}
+
+int mafem_predict(mafem_ctx *ctx, double dt) {
+  ABI_GUARD(ctx, {
+    MAFEM_HIP(hipSetDevice(c.dev));
+    launch_predict(c, c.d_cps, dt);
+    MAFEM_HIP(hipStreamSynchronize(c.stream));
+    return MAFEM_OK;
+  });
+}
+
+int mafem_nnz(mafem_ctx *ctx, int64_t *nnz) {
+  ABI_GUARD(ctx, {
+    if (!nnz) return fail(c, "null nnz", MAFEM_EINVAL);
+    *nnz = c.nnz;
+    return MAFEM_OK;
+  });
+}
+
+int mafem_assemble(mafem_ctx *ctx, double time, double dt, double *r, int64_t *colptr,
+                   int64_t *rowval, double *nzval) {
+  ABI_GUARD(ctx, {
+    MAFEM_HIP(hipSetDevice(c.dev));
+    MAFEM_HIP(hipEventRecord(c.ev[0], c.stream));
+    launch_assemble(c, time, dt);
+    MAFEM_HIP(hipEventRecord(c.ev[1], c.stream));
+    if (r) {
+      hipLaunchKernelGGL(k_gather_idx, dim3(nb256(c.nmdf)), dim3(256), 0, c.stream, c.nmdf, c.d_eq,
+                         c.d_r, c.d_w);
+      MAFEM_HIP(hipGetLastError());
+      MAFEM_HIP(hipMemcpyAsync(r, c.d_w, sizeof(double) * c.nmdf, hipMemcpyDeviceToHost, c.stream));
+    }
+    if (nzval) {
+      hipLaunchKernelGGL(k_gather_idx, dim3(nb256(c.nnz)), dim3(256), 0, c.stream, c.nnz,
+                         c.d_csc_src, c.d_bsr, c.d_csc_val);
+      MAFEM_HIP(hipGetLastError());
+      MAFEM_HIP(hipMemcpyAsync(nzval, c.d_csc_val, sizeof(double) * c.nnz, hipMemcpyDeviceToHost, c.stream));
+    }
+    MAFEM_HIP(hipStreamSynchronize(c.stream));
+    if (colptr) std::memcpy(colptr, c.h_colptr.data(), sizeof(int64_t) * (c.nmdf + 1));
+    if (rowval) std::memcpy(rowval, c.h_rowval.data(), sizeof(int64_t) * c.nnz);
+    c.stats = mafem_stats{};
+    c.stats.assemble_us = 1e3 * elapsed(c, 0, 1);
+    return MAFEM_OK;
+  });
+}
+
+int mafem_solve(mafem_ctx *ctx, const double *rhs, double *du, double *relres) {
+  ABI_GUARD(ctx, {
+    if (!rhs || !du) return fail(c, "null vector", MAFEM_EINVAL);
+    MAFEM_HIP(hipSetDevice(c.dev));
+    MAFEM_HIP(hipMemcpy(c.d_t, rhs, sizeof(double) * c.nmdf, hipMemcpyHostToDevice));
+    vec_zero(c, c.nt * TILE, c.d_b);
+    hipLaunchKernelGGL(k_scatter_idx, dim3(nb256(c.nmdf)), dim3(256), 0, c.stream, c.nmdf, c.d_eq,
+                       c.d_t, c.d_b);
+    MAFEM_HIP(hipGetLastError());
+    MAFEM_HIP(hipEventRecord(c.ev[2], c.stream));
+    band_fill(c);
+    band_factor(c);
+    MAFEM_HIP(hipEventRecord(c.ev[3], c.stream));
+    const double rel = solve_refined(c, c.d_b, c.d_x, 3, 1e-14);
+    MAFEM_HIP(hipEventRecord(c.ev[4], c.stream));
+    hipLaunchKernelGGL(k_gather_idx, dim3(nb256(c.nmdf)), dim3(256), 0, c.stream, c.nmdf, c.d_eq,
+                       c.d_x, c.d_t);
+    MAFEM_HIP(hipGetLastError());
+    MAFEM_HIP(hipMemcpyAsync(du, c.d_t, sizeof(double) * c.nmdf, hipMemcpyDeviceToHost, c.stream));
+    MAFEM_HIP(hipStreamSynchronize(c.stream));
+    if (relres) *relres = rel;
+    c.stats.factor_us = 1e3 * elapsed(c, 2, 3);
+    c.stats.solve_us = 1e3 * elapsed(c, 3, 4);
+    return MAFEM_OK;
+  });
+}
+
+int mafem_time_step(mafem_ctx *ctx, double time, double dt, int32_t max_iter, int32_t *iters,
+                    double *eps_hist) {
+  ABI_GUARD(ctx, {
+    if (max_iter < 1) return fail(c, "max_iter must be >= 1", MAFEM_EINVAL);
+    MAFEM_HIP(hipSetDevice(c.dev));
+    mafem_stats st{};
+    double eps = INFINITY;
+    int it = 0;
+    MAFEM_HIP(hipEventRecord(c.ev[0], c.stream));
+    while (it < max_iter) {
+      MAFEM_HIP(hipEventRecord(c.ev[1], c.stream));
+      launch_assemble(c, time, dt);
+      MAFEM_HIP(hipEventRecord(c.ev[2], c.stream));
+      band_fill(c);
+      band_factor(c);
+      MAFEM_HIP(hipEventRecord(c.ev[3], c.stream));
+      vec_scale(c, c.nt * TILE, -1.0, c.d_r, c.d_b);  // b = -r
+      solve_refined(c, c.d_b, c.d_x, 3, 1e-14);
+      MAFEM_HIP(hipEventRecord(c.ev[4], c.stream));
+      apply_update(c, c.d_x, dt);
+      MAFEM_HIP(hipEventRecord(c.ev[5], c.stream));
+      eps = vec_norm2_masked(c, c.d_x) / (double)c.nmdf;  // synchronizes
+      st.assemble_us += 1e3 * elapsed(c, 1, 2);
+      st.factor_us += 1e3 * elapsed(c, 2, 3);
+      st.solve_us += 1e3 * elapsed(c, 3, 4);
+      st.update_us += 1e3 * elapsed(c, 4, 5);
+      st.refine_sweeps += c.stats.refine_sweeps;
+      st.last_relres = c.stats.last_relres;
+      if (eps_hist) eps_hist[it] = eps;
+      ++it;
+      if (!std::isfinite(eps)) break;
+      if (eps < c.enr) break;
+    }
+    MAFEM_HIP(hipEventRecord(c.ev[6], c.stream));
+    MAFEM_HIP(hipEventSynchronize(c.ev[6]));
+    st.total_us = 1e3 * elapsed(c, 0, 6);
+    st.newton_iters = it;
+    c.stats = st;
+    if (iters) *iters = it;
+    if (!(eps < c.enr)) {
+      c.err = "did not reach Newton--Raphson tolerance";
+      return MAFEM_NOT_CONVERGED;
+    }
+    return MAFEM_OK;
+  });
+}
+
+int mafem_get_stats(const mafem_ctx *ctx, mafem_stats *out) {
+  if (!ctx || !out) return MAFEM_EINVAL;
+  *out = ctx->c.stats;
+  return MAFEM_OK;
+}
+
+}  // extern "C"

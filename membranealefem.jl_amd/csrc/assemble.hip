@@ -1,0 +1,601 @@
+// Element residual/tangent assembly for gfx950.
+//
+// One lane per (element, tangent column): lanes carry forward-mode dual
+// numbers seeded with that column's direction, so each lane produces one
+// column of K_el (all 9 ndf rows) and the residual entry of its own row.
+// Element state and the tabulated basis records of the elements a workgroup
+// touches are staged in LDS once; K_el is written row-major so a store
+// instruction writes one contiguous row across the lanes.
+//
+// Physics follows src/analysis/GeoDynStress.jl:91-178 and
+// src/analysis/FiniteElement.jl:253-330 (area), :338-400 and :431-452 (edges).
+// The tangent columns follow FiniteElement.jl:110-126: a column of a motion
+// DOF is d r/d cps + Δt d r/d xms, i.e. one directional derivative with seed
+// (e_col, Δt e_(node,comp)).  Columns of prescribed DOFs (ID == 0) are zero.
+#include "internal.h"
+
+namespace mafem {
+
+struct D {
+  double v, t;
+};
+__device__ __forceinline__ D mk(double v, double t) { return D{v, t}; }
+__device__ __forceinline__ D operator+(D a, D b) { return D{a.v + b.v, a.t + b.t}; }
+__device__ __forceinline__ D operator-(D a, D b) { return D{a.v - b.v, a.t - b.t}; }
+__device__ __forceinline__ D operator-(D a) { return D{-a.v, -a.t}; }
+__device__ __forceinline__ D operator*(D a, D b) { return D{a.v * b.v, fma(a.v, b.t, a.t * b.v)}; }
+__device__ __forceinline__ D operator*(double s, D a) { return D{s * a.v, s * a.t}; }
+__device__ __forceinline__ D operator*(D a, double s) { return D{s * a.v, s * a.t}; }
+__device__ __forceinline__ D recip(D a) {
+  double r = 1.0 / a.v;
+  return D{r, -a.t * r * r};
+}
+__device__ __forceinline__ D dsqrt(D a) {
+  double s = sqrt(a.v);
+  return D{s, a.t / (2.0 * s)};
+}
+
+// DOF layout of mesh.dofs for each motion (get_dofs, src/input/Bc.jl:414-431)
+template <int MO>
+struct Lay {
+  static constexpr int NDF = (MO == LAG || MO == STATIC) ? 4 : (MO == EUL ? 7 : 8);
+  static constexpr int NE = 9 * NDF;
+  static constexpr bool HAS_VM = (MO == EUL || MO == ALEV || MO == ALEVB);
+  static constexpr bool HAS_PM = (MO == ALEV || MO == ALEVB);
+  static constexpr int LAMI = (MO == LAG || MO == STATIC) ? 3 : 6;
+  static constexpr int PMI = 7;
+  // first column of the motion DOFs (get_m_motion_order, Mesh.jl:529-542)
+  static constexpr int MOT0 = (MO == LAG) ? 0 : (MO == STATIC ? -1 : 3);
+  static constexpr int MAXE = 256 / NE + 2;  // elements a 256-lane block can touch
+};
+
+// Geometry of one Gauss point with the lane's seed (GeoDynStress.jl:111-130).
+struct Geo {
+  D a1[3], a2[3], n[3], au1[3], au2[3], J;
+  D g00, g01, g11;  // a^{αβ}
+  D idet;           // 1 / det a_{αβ}
+};
+
+__device__ __forceinline__ void geometry(const double *X /*[3][9]*/, const double *bN1,
+                                         const double *bN2, int a0, int i0, double sX,
+                                         Geo &g) {
+  const double t1 = sX * bN1[a0], t2 = sX * bN2[a0];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    double s1 = 0, s2 = 0;
+#pragma unroll
+    for (int a = 0; a < 9; ++a) {
+      s1 = fma(X[i * 9 + a], bN1[a], s1);
+      s2 = fma(X[i * 9 + a], bN2[a], s2);
+    }
+    g.a1[i] = mk(s1, i == i0 ? t1 : 0.0);
+    g.a2[i] = mk(s2, i == i0 ? t2 : 0.0);
+  }
+  D a11 = g.a1[0] * g.a1[0] + g.a1[1] * g.a1[1] + g.a1[2] * g.a1[2];
+  D a12 = g.a1[0] * g.a2[0] + g.a1[1] * g.a2[1] + g.a1[2] * g.a2[2];
+  D a22 = g.a2[0] * g.a2[0] + g.a2[1] * g.a2[1] + g.a2[2] * g.a2[2];
+  D det = a11 * a22 - a12 * a12;
+  g.idet = recip(det);
+  g.g00 = a22 * g.idet;
+  g.g01 = -(a12 * g.idet);
+  g.g11 = a11 * g.idet;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    g.au1[i] = g.a1[i] * g.g00 + g.a2[i] * g.g01;
+    g.au2[i] = g.a1[i] * g.g01 + g.a2[i] * g.g11;
+  }
+  g.J = dsqrt(det);
+  D iJ = recip(g.J);
+  g.n[0] = (g.a1[1] * g.a2[2] - g.a1[2] * g.a2[1]) * iJ;
+  g.n[1] = (g.a1[2] * g.a2[0] - g.a1[0] * g.a2[2]) * iJ;
+  g.n[2] = (g.a1[0] * g.a2[1] - g.a1[1] * g.a2[0]) * iJ;
+}
+
+template <int MO>
+__global__ __launch_bounds__(256) void k_area(
+    int64_t numel, int64_t numnp, const int32_t *__restrict__ ix,
+    const int32_t *__restrict__ uel, const uint8_t *__restrict__ active,
+    const double *__restrict__ xms, const double *__restrict__ cps,
+    const double *__restrict__ ufns, const double *__restrict__ dbt, PhysParams ph,
+    double dt, double *__restrict__ kel, double *__restrict__ rel) {
+  using L = Lay<MO>;
+  constexpr int NDF = L::NDF, NE = L::NE, MAXE = L::MAXE;
+  __shared__ double sX[MAXE][3 * 9];
+  __shared__ double sC[MAXE][NDF * 9];
+  __shared__ double sB[MAXE][9 * RECP];
+
+  const int64_t f0 = (int64_t)blockIdx.x * 256;
+  const int64_t e_lo = f0 / NE;
+  int64_t e_hi = (f0 + 255) / NE;
+  if (e_hi > numel - 1) e_hi = numel - 1;
+  const int nle = (int)(e_hi - e_lo + 1);
+  // stage element state: positions and control points of the 9 nodes
+  for (int k = threadIdx.x; k < nle * 9 * (3 + NDF); k += 256) {
+    const int le = k / (9 * (3 + NDF)), rem = k % (9 * (3 + NDF));
+    const int comp = rem / 9, a = rem % 9;
+    const int64_t node = ix[(e_lo + le) * 9 + a];
+    if (comp < 3)
+      sX[le][comp * 9 + a] = xms[node + numnp * comp];
+    else
+      sC[le][(comp - 3) * 9 + a] = cps[node + numnp * (comp - 3)];
+  }
+  // stage the 9 Gauss-point basis records of each element
+  for (int k = threadIdx.x; k < nle * 9 * RECP; k += 256) {
+    const int le = k / (9 * RECP), rem = k % (9 * RECP);
+    sB[le][rem] = ufns[(int64_t)uel[e_lo + le] * 9 * RECP + rem];
+  }
+  __syncthreads();
+
+  const int64_t f = f0 + threadIdx.x;
+  const bool valid = f < numel * NE;
+  const int64_t e = valid ? f / NE : e_hi;
+  const int col = (int)(f % NE);
+  const int le = (int)(e - e_lo);
+  const int a0 = col / NDF, d0 = col % NDF;
+  const int64_t node0 = ix[e * 9 + a0];
+  const bool act = active[d0 + NDF * node0] != 0;
+  const double sC0 = act ? 1.0 : 0.0;
+  const bool mot = (L::MOT0 >= 0) && d0 >= L::MOT0 && d0 < L::MOT0 + 3;
+  const int i0 = mot ? d0 - L::MOT0 : -1;
+  const double sX0 = (act && mot) ? dt : 0.0;
+  const double *X = sX[le];
+  const double *C = sC[le];
+
+  double acc[NE];
+#pragma unroll
+  for (int k = 0; k < NE; ++k) acc[k] = 0.0;
+  double racc = 0.0;
+  const double azv = ph.adb / ph.zv;
+
+  for (int gp = 0; gp < 9; ++gp) {
+    const double *rec = &sB[le][gp * RECP];
+    const double w = rec[0];
+    const double *bN = rec + 1, *bN1 = rec + 10, *bN2 = rec + 19;
+    const double *b11 = rec + 28, *b22 = rec + 37, *b12 = rec + 46;
+
+    Geo g;
+    geometry(X, bN1, bN2, a0, i0, sX0, g);
+    // second derivatives of position: x_{,11}, x_{,22}, x_{,12}
+    D x11[3], x22[3], x12[3];
+    {
+      const double t11 = sX0 * b11[a0], t22 = sX0 * b22[a0], t12 = sX0 * b12[a0];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        double s11 = 0, s22 = 0, s12 = 0;
+#pragma unroll
+        for (int a = 0; a < 9; ++a) {
+          s11 = fma(X[i * 9 + a], b11[a], s11);
+          s22 = fma(X[i * 9 + a], b22[a], s22);
+          s12 = fma(X[i * 9 + a], b12[a], s12);
+        }
+        x11[i] = mk(s11, i == i0 ? t11 : 0.0);
+        x22[i] = mk(s22, i == i0 ? t22 : 0.0);
+        x12[i] = mk(s12, i == i0 ? t12 : 0.0);
+      }
+    }
+    // Christoffel symbols Γ[v][μ] = x_{,v} · a^μ (v = 11, 22, 12)
+    D G0[3], G1[3];
+    G0[0] = x11[0] * g.au1[0] + x11[1] * g.au1[1] + x11[2] * g.au1[2];
+    G0[1] = x22[0] * g.au1[0] + x22[1] * g.au1[1] + x22[2] * g.au1[2];
+    G0[2] = x12[0] * g.au1[0] + x12[1] * g.au1[1] + x12[2] * g.au1[2];
+    G1[0] = x11[0] * g.au2[0] + x11[1] * g.au2[1] + x11[2] * g.au2[2];
+    G1[1] = x22[0] * g.au2[0] + x22[1] * g.au2[1] + x22[2] * g.au2[2];
+    G1[2] = x12[0] * g.au2[0] + x12[1] * g.au2[1] + x12[2] * g.au2[2];
+    // curvature b_{αβ}, b^{αβ}, H, K
+    D bb11 = x11[0] * g.n[0] + x11[1] * g.n[1] + x11[2] * g.n[2];
+    D bb22 = x22[0] * g.n[0] + x22[1] * g.n[1] + x22[2] * g.n[2];
+    D bb12 = x12[0] * g.n[0] + x12[1] * g.n[1] + x12[2] * g.n[2];
+    D tb00 = g.g00 * bb11 + g.g01 * bb12, tb01 = g.g00 * bb12 + g.g01 * bb22;
+    D tb10 = g.g01 * bb11 + g.g11 * bb12, tb11 = g.g01 * bb12 + g.g11 * bb22;
+    D B00 = tb00 * g.g00 + tb01 * g.g01, B01 = tb00 * g.g01 + tb01 * g.g11;
+    D B10 = tb10 * g.g00 + tb11 * g.g01, B11 = tb10 * g.g01 + tb11 * g.g11;
+    D H = 0.5 * (g.g00 * bb11 + g.g01 * bb12 + g.g01 * bb12 + g.g11 * bb22);
+    D Kc = (bb11 * bb22 - bb12 * bb12) * g.idet;
+
+    // fields: v, v_{,α}, λ, and (EUL/ALE) v^m, v^m_{,α}, p^m
+    D v[3], dv0[3], dv1[3];
+    const double tN = sC0 * bN[a0], tN1 = sC0 * bN1[a0], tN2 = sC0 * bN2[a0];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double s = 0, s1 = 0, s2 = 0;
+#pragma unroll
+      for (int a = 0; a < 9; ++a) {
+        const double c = C[i * 9 + a];
+        s = fma(c, bN[a], s);
+        s1 = fma(c, bN1[a], s1);
+        s2 = fma(c, bN2[a], s2);
+      }
+      const bool sd = (d0 == i);
+      v[i] = mk(s, sd ? tN : 0.0);
+      dv0[i] = mk(s1, sd ? tN1 : 0.0);
+      dv1[i] = mk(s2, sd ? tN2 : 0.0);
+    }
+    D lam;
+    {
+      double s = 0;
+#pragma unroll
+      for (int a = 0; a < 9; ++a) s = fma(C[L::LAMI * 9 + a], bN[a], s);
+      lam = mk(s, d0 == L::LAMI ? tN : 0.0);
+    }
+    D vm[3], dvm0[3], dvm1[3], pm = mk(0, 0);
+    if constexpr (L::HAS_VM) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        double s = 0, s1 = 0, s2 = 0;
+#pragma unroll
+        for (int a = 0; a < 9; ++a) {
+          const double c = C[(3 + i) * 9 + a];
+          s = fma(c, bN[a], s);
+          s1 = fma(c, bN1[a], s1);
+          s2 = fma(c, bN2[a], s2);
+        }
+        const bool sd = (d0 == 3 + i);
+        vm[i] = mk(s, sd ? tN : 0.0);
+        dvm0[i] = mk(s1, sd ? tN1 : 0.0);
+        dvm1[i] = mk(s2, sd ? tN2 : 0.0);
+      }
+    }
+    if constexpr (L::HAS_PM) {
+      double s = 0;
+#pragma unroll
+      for (int a = 0; a < 9; ++a) s = fma(C[L::PMI * 9 + a], bN[a], s);
+      pm = mk(s, d0 == L::PMI ? tN : 0.0);
+    }
+
+    // stresses (GeoDynStress.jl:148-159), Voigt [11, 22, 2*12] / [11, 22, 12+21]
+    D sg[3], Mv[3], sgm[3];
+    {
+      D w0[3], w1[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        w0[i] = dv0[i] * g.g00 + dv1[i] * g.g01;
+        w1[i] = dv0[i] * g.g01 + dv1[i] * g.g11;
+      }
+      D p00 = g.au1[0] * w0[0] + g.au1[1] * w0[1] + g.au1[2] * w0[2];
+      D p01 = g.au1[0] * w1[0] + g.au1[1] * w1[1] + g.au1[2] * w1[2];
+      D p10 = g.au2[0] * w0[0] + g.au2[1] * w0[1] + g.au2[2] * w0[2];
+      D p11 = g.au2[0] * w1[0] + g.au2[1] * w1[1] + g.au2[2] * w1[2];
+      D c0 = ph.kb * (H * H) - ph.kg * Kc;
+      D lc = c0 + lam;
+      D h2 = (2.0 * ph.kb) * H;
+      sg[0] = g.g00 * lc - B00 * h2 + (2.0 * ph.zv) * p00;
+      sg[1] = g.g11 * lc - B11 * h2 + (2.0 * ph.zv) * p11;
+      sg[2] = 2.0 * (g.g01 * lc - B01 * h2 + ph.zv * (p01 + p10));
+      D kk = H * (ph.kb + 2.0 * ph.kg);
+      Mv[0] = g.g00 * kk - B00 * ph.kg;
+      Mv[1] = g.g11 * kk - B11 * ph.kg;
+      Mv[2] = (g.g01 * kk - B01 * ph.kg) + (g.g01 * kk - B10 * ph.kg);
+      if constexpr (L::HAS_PM) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          w0[i] = dvm0[i] * g.g00 + dvm1[i] * g.g01;
+          w1[i] = dvm0[i] * g.g01 + dvm1[i] * g.g11;
+        }
+        D q00 = g.au1[0] * w0[0] + g.au1[1] * w0[1] + g.au1[2] * w0[2];
+        D q01 = g.au1[0] * w1[0] + g.au1[1] * w1[1] + g.au1[2] * w1[2];
+        D q10 = g.au2[0] * w0[0] + g.au2[1] * w0[1] + g.au2[2] * w0[2];
+        D q11 = g.au2[0] * w1[0] + g.au2[1] * w1[1] + g.au2[2] * w1[2];
+        sgm[0] = g.g00 * c0 - B00 * h2 + (2.0 * ph.zv) * q00;
+        sgm[1] = g.g11 * c0 - B11 * h2 + (2.0 * ph.zv) * q11;
+        sgm[2] = 2.0 * (g.g01 * c0 - B01 * h2 + ph.zv * (q01 + q10));
+      }
+    }
+
+    // per-Gauss-point factors of the residual rows
+    const D wJ = w * g.J;
+    D A1[3], A2[3], Nn[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      A1[i] = wJ * g.a1[i];
+      A2[i] = wJ * g.a2[i];
+      Nn[i] = wJ * g.n[i];
+    }
+    D tr = g.au1[0] * dv0[0] + g.au1[1] * dv0[1] + g.au1[2] * dv0[2] +
+           g.au2[0] * dv1[0] + g.au2[1] * dv1[1] + g.au2[2] * dv1[2];
+    const D ql = w * (g.J * tr - azv * lam);
+    D gm[3], qp = mk(0, 0), pmJ = mk(0, 0);
+    if constexpr (MO == EUL) {
+      D nv = g.n[0] * v[0] + g.n[1] * v[1] + g.n[2] * v[2];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) gm[i] = (ph.am * wJ) * (vm[i] - g.n[i] * nv);
+    }
+    if constexpr (L::HAS_PM) {
+      D nd = g.n[0] * (vm[0] - v[0]) + g.n[1] * (vm[1] - v[1]) + g.n[2] * (vm[2] - v[2]);
+      qp = -(wJ * nd) - (w * azv) * pm;
+      pmJ = pm * wJ;
+    }
+
+#define ADD(row, val)                          \
+  do {                                         \
+    const D v_ = (val);                        \
+    acc[(row)] += v_.t;                        \
+    racc += ((row) == col) ? v_.v : 0.0;       \
+  } while (0)
+
+#pragma unroll
+    for (int a = 0; a < 9; ++a) {
+      const double n1 = bN1[a], n2 = bN2[a], na = bN[a];
+      D s1 = n1 * sg[0] + (0.5 * n2) * sg[2];
+      D s2 = n2 * sg[1] + (0.5 * n1) * sg[2];
+      D nn0 = mk(b11[a], 0.0) - (n1 * G0[0] + n2 * G1[0]);
+      D nn1 = mk(b22[a], 0.0) - (n1 * G0[1] + n2 * G1[1]);
+      D nn2 = mk(b12[a], 0.0) - (n1 * G0[2] + n2 * G1[2]);
+      D m = nn0 * Mv[0] + nn1 * Mv[1] + nn2 * Mv[2];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        D rv = A1[i] * s1 + A2[i] * s2 + Nn[i] * m;
+        if (ph.pn != 0.0) rv = rv - (ph.pn * na) * Nn[i];
+        ADD(a * NDF + i, rv);
+      }
+      ADD(a * NDF + L::LAMI, na * ql);
+      if constexpr (MO == EUL) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) ADD(a * NDF + 3 + i, na * gm[i]);
+      }
+      if constexpr (L::HAS_PM) {
+        D t1 = n1 * sgm[0] + (0.5 * n2) * sgm[2];
+        D t2 = n2 * sgm[1] + (0.5 * n1) * sgm[2];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          D rm = A1[i] * t1 + A2[i] * t2;
+          if constexpr (MO == ALEVB) rm = rm + Nn[i] * m;
+          rm = rm - na * (pmJ * g.n[i]);
+          ADD(a * NDF + 3 + i, rm);
+        }
+        ADD(a * NDF + L::PMI, na * qp);
+      }
+    }
+#undef ADD
+  }
+
+  // Dohrmann--Bochev projection (FiniteElement.jl:315-327): linear in λ, p^m
+  {
+    const double *T = dbt + (int64_t)uel[e] * 81;
+    if (d0 == L::LAMI && act) {
+#pragma unroll
+      for (int a = 0; a < 9; ++a) acc[a * NDF + L::LAMI] += T[a * 9 + a0] * azv;
+    }
+    if constexpr (L::HAS_PM) {
+      if (d0 == L::PMI && act) {
+#pragma unroll
+        for (int a = 0; a < 9; ++a) acc[a * NDF + L::PMI] += T[a * 9 + a0] * azv;
+      }
+    }
+    if (d0 == L::LAMI || (L::HAS_PM && d0 == L::PMI)) {
+      double s = 0;
+      for (int b = 0; b < 9; ++b) s = fma(T[a0 * 9 + b], C[d0 * 9 + b], s);
+      racc += s * azv;
+    }
+  }
+
+  if (valid) {
+    double *K = kel + e * (int64_t)NE * NE;
+#pragma unroll
+    for (int row = 0; row < NE; ++row) K[row * NE + col] = acc[row];
+    rel[e * NE + col] = racc;
+  }
+}
+
+// Neumann edge elements (FiniteElement.jl:338-400): 3 Gauss points, v rows only.
+template <int MO>
+__global__ __launch_bounds__(256) void k_edge(
+    int64_t n_items, int64_t numnp, const EdgeItem *__restrict__ items,
+    const int32_t *__restrict__ ix, const uint8_t *__restrict__ active,
+    const double *__restrict__ xms, const double *__restrict__ cps,
+    const double *__restrict__ bufns, PhysParams ph, double time, double dt,
+    double *__restrict__ kel, double *__restrict__ rel) {
+  using L = Lay<MO>;
+  constexpr int NDF = L::NDF, NE = L::NE, MAXE = L::MAXE;
+  __shared__ double sX[MAXE][3 * 9];
+  __shared__ double sB[MAXE][3 * RECP];
+
+  const int64_t f0 = (int64_t)blockIdx.x * 256;
+  const int64_t k_lo = f0 / NE;
+  int64_t k_hi = (f0 + 255) / NE;
+  if (k_hi > n_items - 1) k_hi = n_items - 1;
+  const int nle = (int)(k_hi - k_lo + 1);
+  for (int k = threadIdx.x; k < nle * 27; k += 256) {
+    const int le = k / 27, rem = k % 27, comp = rem / 9, a = rem % 9;
+    const int64_t node = ix[(int64_t)items[k_lo + le].el * 9 + a];
+    sX[le][comp * 9 + a] = xms[node + numnp * comp];
+  }
+  for (int k = threadIdx.x; k < nle * 3 * RECP; k += 256) {
+    const int le = k / (3 * RECP), rem = k % (3 * RECP);
+    sB[le][rem] = bufns[(int64_t)items[k_lo + le].rec * RECP + rem];
+  }
+  __syncthreads();
+
+  const int64_t f = f0 + threadIdx.x;
+  const bool valid = f < n_items * NE;
+  const int64_t k = valid ? f / NE : k_hi;
+  const int col = (int)(f % NE);
+  const int le = (int)(k - k_lo);
+  const EdgeItem it = items[k];
+  const int a0 = col / NDF, d0 = col % NDF;
+  const int64_t node0 = ix[(int64_t)it.el * 9 + a0];
+  const bool act = active[d0 + NDF * node0] != 0;
+  const bool mot = (L::MOT0 >= 0) && d0 >= L::MOT0 && d0 < L::MOT0 + 3;
+  const int i0 = mot ? d0 - L::MOT0 : -1;
+  const double sX0 = (act && mot) ? dt : 0.0;
+  const double *X = sX[le];
+
+  double acc[27];
+#pragma unroll
+  for (int q = 0; q < 27; ++q) acc[q] = 0.0;
+  double racc = 0.0;
+  const double mval = (it.ntype == MOMENT) ? it.nval * fmin(time / ph.bend_tm, 1.0) : 0.0;
+
+  for (int gp = 0; gp < 3; ++gp) {
+    const double *rec = &sB[le][gp * RECP];
+    const double w = rec[0];
+    const double *bN = rec + 1, *bN1 = rec + 10, *bN2 = rec + 19;
+    Geo g;
+    geometry(X, bN1, bN2, a0, i0, sX0, g);
+    // edge frame (calc_τ_ν): τ = ±a_1 / ±a_2, normalised by the conjugating
+    // dot product, whose complex-step derivative is zero: only τ's own
+    // variation is carried, divided by the primal norm.
+    D tau[3];
+    const double sgn = (it.bdry == TOP || it.bdry == LEFT) ? -1.0 : 1.0;
+    const bool along1 = (it.bdry == BOTTOM || it.bdry == TOP);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) tau[i] = sgn * (along1 ? g.a1[i] : g.a2[i]);
+    const double tn = sqrt(tau[0].v * tau[0].v + tau[1].v * tau[1].v + tau[2].v * tau[2].v);
+    const double itn = 1.0 / tn;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) tau[i] = itn * tau[i];
+    D nu[3];
+    nu[0] = tau[1] * g.n[2] - tau[2] * g.n[1];
+    nu[1] = tau[2] * g.n[0] - tau[0] * g.n[2];
+    nu[2] = tau[0] * g.n[1] - tau[1] * g.n[0];
+    D q0 = g.au1[0] * tau[0] + g.au1[1] * tau[1] + g.au1[2] * tau[2];
+    D q1 = g.au2[0] * tau[0] + g.au2[1] * tau[1] + g.au2[2] * tau[2];
+    const D JGw = w * recip(dsqrt(q0 * q0 + q1 * q1));
+    if (it.ntype == STRETCH || it.ntype == SHEAR) {
+      D fv[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) fv[i] = (it.nval * JGw) * (it.ntype == STRETCH ? nu[i] : tau[i]);
+#pragma unroll
+      for (int a = 0; a < 9; ++a)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          const D val = -(bN[a] * fv[i]);
+          acc[a * 3 + i] += val.t;
+          racc += (a * NDF + i == col) ? val.v : 0.0;
+        }
+    } else {  // MOMENT (validated on the host: F_BEND only)
+      D nu0 = g.au1[0] * nu[0] + g.au1[1] * nu[1] + g.au1[2] * nu[2];
+      D nu1 = g.au2[0] * nu[0] + g.au2[1] * nu[1] + g.au2[2] * nu[2];
+      const D s = mval * JGw;
+#pragma unroll
+      for (int a = 0; a < 9; ++a) {
+        const D dn = bN1[a] * nu0 + bN2[a] * nu1;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          const D val = -((g.n[i] * dn) * s);
+          acc[a * 3 + i] += val.t;
+          racc += (a * NDF + i == col) ? val.v : 0.0;
+        }
+      }
+    }
+  }
+  if (valid) {
+    double *K = kel + k * (int64_t)NE * NE;
+#pragma unroll
+    for (int row = 0; row < NE; ++row) {
+      const int a = row / NDF, d = row % NDF;
+      K[row * NE + col] = (d < 3) ? acc[a * 3 + d] : 0.0;
+    }
+    rel[k * NE + col] = racc;
+  }
+}
+
+// Deterministic gather of element blocks into the node-block matrix of the
+// extended system: entries of prescribed DOFs become identity rows/columns.
+__global__ __launch_bounds__(256) void k_gather_bsr(
+    int64_t nnzb, int ndf, int ne, const int32_t *__restrict__ brow,
+    const int32_t *__restrict__ colidx, const uint8_t *__restrict__ active,
+    const int64_t *__restrict__ gptr, const int64_t *__restrict__ gsrc,
+    const double *__restrict__ kel, double *__restrict__ bsr) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int bb = ndf * ndf;
+  if (t >= nnzb * bb) return;
+  const int64_t blk = t / bb;
+  const int rc = (int)(t % bb), r = rc / ndf, c = rc % ndf;
+  const int64_t p = brow[blk], q = colidx[blk];
+  const bool ar = active[r + (int64_t)ndf * p] != 0, ac = active[c + (int64_t)ndf * q] != 0;
+  double s = 0.0;
+  if (ar && ac) {
+    const int64_t off = (int64_t)r * ne + c;
+    for (int64_t k = gptr[blk]; k < gptr[blk + 1]; ++k) s += kel[gsrc[k] + off];
+  } else if (!ar && p == q && r == c) {
+    s = 1.0;
+  }
+  bsr[t] = s;
+}
+
+__global__ __launch_bounds__(256) void k_gather_r(
+    int64_t nx, int ndf, const uint8_t *__restrict__ active,
+    const int64_t *__restrict__ rptr, const int64_t *__restrict__ rsrc,
+    const double *__restrict__ rel, double *__restrict__ r) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= nx) return;
+  const int64_t p = t / ndf;
+  const int d = (int)(t % ndf);
+  double s = 0.0;
+  if (active[t]) {
+    for (int64_t k = rptr[p]; k < rptr[p + 1]; ++k) s += rel[rsrc[k] + d];
+  }
+  r[t] = s;
+}
+
+// x += Δt u[:, motion] over all nodes (update_xms!, FiniteElement.jl:408-423);
+// u is column-major numnp x ndf (the predictor passes cps).
+__global__ void k_predict(int64_t numnp, int m0, int m1, int m2, const double *__restrict__ u,
+                          double dt, double *__restrict__ xms) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= numnp) return;
+  const int mm[3] = {m0, m1, m2};
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    if (mm[i] > 0) xms[p + numnp * i] += dt * u[p + numnp * (mm[i] - 1)];
+}
+
+static hipError_t launch_area(Ctx &c, double dt) {
+  const int64_t lanes = c.numel * c.ne;
+  const dim3 grid((unsigned)((lanes + 255) / 256));
+#define AREA(MO)                                                                        \
+  hipLaunchKernelGGL(k_area<MO>, grid, dim3(256), 0, c.stream, c.numel, c.numnp, c.d_ix, \
+                     c.d_uel, c.d_active, c.d_xms, c.d_cps, c.d_ufns, c.d_dbt, c.phys, dt, \
+                     c.d_kel, c.d_rel)
+  switch (c.motion) {
+    case LAG: AREA(LAG); break;
+    case EUL: AREA(EUL); break;
+    case ALEV: AREA(ALEV); break;
+    case ALEVB: AREA(ALEVB); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef AREA
+  return hipGetLastError();
+}
+
+static hipError_t launch_edge(Ctx &c, double time, double dt) {
+  if (c.n_edges == 0) return hipSuccess;
+  const int64_t lanes = c.n_edges * c.ne;
+  const dim3 grid((unsigned)((lanes + 255) / 256));
+  double *kel = c.d_kel + c.numel * (int64_t)c.ne * c.ne;
+  double *rel = c.d_rel + c.numel * (int64_t)c.ne;
+#define EDGE(MO)                                                                            \
+  hipLaunchKernelGGL(k_edge<MO>, grid, dim3(256), 0, c.stream, c.n_edges, c.numnp, c.d_edges, \
+                     c.d_ix, c.d_active, c.d_xms, c.d_cps, c.d_bufns, c.phys, time, dt, kel, rel)
+  switch (c.motion) {
+    case LAG: EDGE(LAG); break;
+    case EUL: EDGE(EUL); break;
+    case ALEV: EDGE(ALEV); break;
+    case ALEVB: EDGE(ALEVB); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef EDGE
+  return hipGetLastError();
+}
+
+int launch_assemble(Ctx &c, double time, double dt) {
+  MAFEM_HIP(launch_area(c, dt));
+  MAFEM_HIP(launch_edge(c, time, dt));
+  const int64_t nb = c.nnzb * c.ndf * c.ndf;
+  hipLaunchKernelGGL(k_gather_bsr, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, c.stream,
+                     c.nnzb, c.ndf, c.ne, c.d_brow, c.d_colidx, c.d_active, c.d_gptr, c.d_gsrc,
+                     c.d_kel, c.d_bsr);
+  MAFEM_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_gather_r, dim3((unsigned)((c.nx + 255) / 256)), dim3(256), 0, c.stream,
+                     c.nx, c.ndf, c.d_active, c.d_rptr, c.d_rsrc, c.d_rel, c.d_r);
+  MAFEM_HIP(hipGetLastError());
+  return 0;
+}
+
+void launch_predict(Ctx &c, const double *d_u, double dt) {
+  hipLaunchKernelGGL(k_predict, dim3((unsigned)((c.numnp + 255) / 256)), dim3(256), 0, c.stream,
+                     c.numnp, c.mm_order[0], c.mm_order[1], c.mm_order[2], d_u, dt, c.d_xms);
+  MAFEM_HIP(hipGetLastError());
+}
+
+}  // namespace mafem

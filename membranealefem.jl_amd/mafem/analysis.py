@@ -1,0 +1,241 @@
+"""Host mirror of the reference's analysis entry points on the HIP path.
+
+`time_step`, `calc_r_K`, `update_xms` and `run_analysis` keep the reference's
+names, arguments and error behaviour (src/analysis/FiniteElement.jl:11-200,
+408-423; src/Analysis.jl:17-102); the work runs in libmafem on the GPU.
+`DeviceMesh` is the device-resident context behind them (one per mesh).
+"""
+import ctypes
+import os
+
+import numpy as np
+import scipy.sparse as sps
+
+from . import _lib as L
+from .enums import BOTTOM, LEFT, RIGHT, TOP, Unknown
+from .mesh import m_motion_order, m_order, v_order
+from .params import Params
+
+_BDRY = (BOTTOM, RIGHT, TOP, LEFT)
+
+
+class DeviceMesh:
+    """A libmafem context: mesh tables and state resident on the GPU."""
+
+    def __init__(self, mesh, p: Params, bend_tm=1.0):
+        self.lib = L.load()
+        self.mesh, self.p = mesh, p
+        m = mesh
+        keep = self._keep = []
+
+        def a64(x):
+            x = np.ascontiguousarray(np.asarray(x, dtype=np.int64).ravel(order="F"))
+            keep.append(x)
+            return L.ptr(x, L.i64p)
+
+        def af(x):
+            x = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+            keep.append(x)
+            return L.ptr(x, L.f64p)
+
+        d = L.MeshDesc()
+        d.num1el, d.num2el, d.numel = m.num1el, m.num2el, m.numel
+        d.num1np, d.num2np, d.numnp = m.num1np, m.num2np, m.numnp
+        d.ndf, d.nmdf = m.ndf, m.nmdf
+        d.IX, d.ID = a64(m.IX), a64(m.ID)
+        d.v_order[:] = v_order(m.dofs)
+        d.m_order[:] = m_order(m.dofs)
+        d.lam_order = m.dofs.get(Unknown.lam, 0)
+        d.p_order = m.dofs.get(Unknown.pm, 0)
+        d.area_n_uel = m.area_ufns.shape[1]
+        d.area_uel_ids = a64(m.area_uel_ids)
+        d.area_ufns = af(m.area_ufns)
+        for k, b in enumerate(_BDRY):
+            d.bdry_n_el[k] = len(m.bdry_elems[b])
+            d.bdry_elems[k] = a64(m.bdry_elems[b])
+            d.bdry_n_uel[k] = m.bdry_ufns[b].shape[1]
+            d.bdry_uel_ids[k] = a64(m.bdry_uel_ids[b])
+            d.bdry_ufns[k] = af(m.bdry_ufns[b])
+        nb = np.array([int(t[0]) for t in m.inh_neu_bcs], dtype=np.int32)
+        nt = np.array([int(t[1]) for t in m.inh_neu_bcs], dtype=np.int32)
+        nv = np.array([float(t[2]) for t in m.inh_neu_bcs], dtype=np.float64)
+        keep += [nb, nt, nv]
+        d.n_neu = len(nb)
+        d.neu_bdry, d.neu_type, d.neu_val = L.ptr(nb, L.i32p), L.ptr(nt, L.i32p), L.ptr(nv, L.f64p)
+        q = L.ParamsDesc(int(p.motion), int(p.scenario), p.kb, p.kg, p.zv, p.pn, p.adb, p.am,
+                         p.ek, p.enr, bend_tm)
+        h = ctypes.c_void_p()
+        rc = self.lib.mafem_create(ctypes.byref(h), ctypes.byref(d), ctypes.byref(q))
+        if rc != L.OK:
+            raise L.LibError(rc, self.lib.mafem_last_error(None).decode())
+        self.h = h
+        self.nmdf = m.nmdf
+        self._nnz = None
+
+    # -- plumbing
+    def _check(self, rc):
+        if rc not in (L.OK, L.NOT_CONVERGED):
+            raise L.LibError(rc, self.lib.mafem_last_error(self.h).decode())
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mafem_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- state
+    def set_state(self, xms, cps):
+        x = np.asfortranarray(xms, dtype=np.float64)
+        c = np.asfortranarray(cps, dtype=np.float64)
+        self._check(self.lib.mafem_set_state(self.h, L.ptr(x, L.f64p), L.ptr(c, L.f64p)))
+
+    def get_state(self):
+        m = self.mesh
+        x = np.zeros((m.numnp, 3), order="F")
+        c = np.zeros((m.numnp, m.ndf), order="F")
+        self._check(self.lib.mafem_get_state(self.h, L.ptr(x, L.f64p), L.ptr(c, L.f64p)))
+        return x, c
+
+    def set_prescribed(self, nodes, dofs, vals):
+        n = np.ascontiguousarray(nodes, dtype=np.int64)
+        d = np.ascontiguousarray(dofs, dtype=np.int32)
+        v = np.ascontiguousarray(vals, dtype=np.float64)
+        self._check(self.lib.mafem_set_prescribed(self.h, len(n), L.ptr(n, L.i64p),
+                                                  L.ptr(d, L.i32p), L.ptr(v, L.f64p)))
+
+    def predict(self, dt):
+        self._check(self.lib.mafem_predict(self.h, float(dt)))
+
+    # -- hot path
+    def nnz(self):
+        if self._nnz is None:
+            n = ctypes.c_int64()
+            self._check(self.lib.mafem_nnz(self.h, ctypes.byref(n)))
+            self._nnz = n.value
+        return self._nnz
+
+    def assemble(self, time, dt, want_K=True):
+        n, nz = self.nmdf, self.nnz()
+        r = np.zeros(n)
+        if not want_K:
+            self._check(self.lib.mafem_assemble(self.h, time, dt, L.ptr(r, L.f64p), None, None,
+                                                None))
+            return r, None
+        cp = np.zeros(n + 1, dtype=np.int64)
+        rv = np.zeros(nz, dtype=np.int64)
+        v = np.zeros(nz)
+        self._check(self.lib.mafem_assemble(self.h, time, dt, L.ptr(r, L.f64p), L.ptr(cp, L.i64p),
+                                            L.ptr(rv, L.i64p), L.ptr(v, L.f64p)))
+        K = sps.csc_matrix((v, rv - 1, cp - 1), shape=(n, n))
+        return r, K
+
+    def solve(self, rhs):
+        b = np.ascontiguousarray(rhs, dtype=np.float64)
+        x = np.zeros(self.nmdf)
+        rel = ctypes.c_double()
+        self._check(self.lib.mafem_solve(self.h, L.ptr(b, L.f64p), L.ptr(x, L.f64p),
+                                         ctypes.byref(rel)))
+        return x, rel.value
+
+    def time_step(self, time, dt, max_iter=14):
+        it = ctypes.c_int32()
+        eps = np.zeros(max_iter)
+        rc = self._check(self.lib.mafem_time_step(self.h, float(time), float(dt), int(max_iter),
+                                                  ctypes.byref(it), L.ptr(eps, L.f64p)))
+        return rc, list(eps[: it.value])
+
+    def stats(self):
+        s = L.Stats()
+        self._check(self.lib.mafem_get_stats(self.h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in L.Stats._fields_}
+
+
+def device_for(mesh, p, **args):
+    """The DeviceMesh cached on a mesh (rebuilt if the parameters change)."""
+    key = (p.motion, p.scenario, p.kb, p.kg, p.zv, p.pn, p.adb, p.am, p.enr,
+           args.get("bend_tm", 1.0))
+    cur = mesh._ctx
+    if cur is None or cur[0] != key:
+        if cur is not None:
+            cur[1].close()
+        mesh._ctx = (key, DeviceMesh(mesh, p, bend_tm=args.get("bend_tm", 1.0)))
+    return mesh._ctx[1]
+
+
+def update_xms(motion, xms, cps, dt, dofs):
+    """update_xms! (FiniteElement.jl:408-423) on host arrays, in place."""
+    for j, c in enumerate(m_motion_order(motion, dofs)):
+        if c != 0:
+            xms[:, j] += dt * cps[:, c - 1]
+
+
+def calc_r_K(mesh, xms, cps, time, dt, p, **args):
+    """calc_r_K (FiniteElement.jl:75-200): (r_gl, K_gl as scipy CSC)."""
+    dev = device_for(mesh, p, **args)
+    dev.set_state(xms, cps)
+    return dev.assemble(float(time), float(dt))
+
+
+def time_step(mesh, xms, cps, time, dt, p, **args):
+    """time_step! (FiniteElement.jl:11-63): Newton on the GPU; mutates xms, cps.
+
+    Raises AssertionError("did not reach Newton--Raphson tolerance") when the
+    last ε = |Δu|_2 / nmdf is not below p.enr after at most 14 iterations.
+    """
+    dev = device_for(mesh, p, **args)
+    dev.set_state(xms, cps)
+    rc, eps = dev.time_step(time, dt)
+    x, c = dev.get_state()
+    xms[...] = x
+    cps[...] = c
+    if p.output:
+        with open(os.path.join(args["out_path"], args["out_file"]), "a") as io:
+            for k, e in enumerate(eps):
+                io.write(f"--> iteration {k + 1}: ε = {e}\n")
+    if rc == L.NOT_CONVERGED:
+        raise AssertionError("did not reach Newton--Raphson tolerance")
+    return eps
+
+
+def run_analysis(mesh, xms, cps, p, **args):
+    """run_analysis! (Analysis.jl:17-102): predictor + Newton per time step,
+    with the state resident on the GPU; returns the ε history of each step.
+
+    An optional `pull_schedule(t_id, time) -> speed` keyword re-applies the
+    F_PULL prescribed speed before each step (what a restart with a new
+    `pull_speed` does in the reference, src/Input.jl:103-108).
+    """
+    dev = device_for(mesh, p, **args)
+    dts = list(args["dts"])
+    times = np.cumsum(dts) + args.get("t0", 0.0)
+    sched = args.get("pull_schedule")
+    pres = None
+    if sched is not None:
+        pres = [(node, mesh.dofs[u]) for (u, node, _v) in mesh.inh_dir_bcs]
+        if p.motion == 2:  # EUL also prescribes v^m_z at the pulled nodes
+            pres += [(node, mesh.dofs[Unknown.vmz]) for (u, node, _v) in mesh.inh_dir_bcs
+                     if u == Unknown.vz]
+    dev.set_state(xms, cps)
+    hist = []
+    for k, dt in enumerate(dts):
+        if sched is not None:
+            s = float(sched(k, float(times[k])))
+            dev.set_prescribed([n for n, _ in pres], [d for _, d in pres], [s] * len(pres))
+        dev.predict(dt)
+        rc, eps = dev.time_step(float(times[k]), float(dt))
+        hist.append(eps)
+        if rc == L.NOT_CONVERGED:
+            x, c = dev.get_state()
+            xms[...] = x
+            cps[...] = c
+            raise AssertionError("did not reach Newton--Raphson tolerance")
+    x, c = dev.get_state()
+    xms[...] = x
+    cps[...] = c
+    return hist
