@@ -1,0 +1,201 @@
+"""Benchmark of the per-time-step hot path on one MI355X.
+
+Workload (the plain FP64 GPU configuration): flat square patch of side 1,
+64 x 64 biquadratic B-spline elements, ALE (ALEVB) formulation, kb = 1,
+ζv = 1, kg = -0.5, tether pull on the pulled element ramped linearly to 2.0
+over 200 steps of dt = 1e-3, seeded N(0, 1e-6) out-of-plane perturbation of
+the control points, Newton tolerance 1e-10.  (With N(0, 1e-4) the reference
+algorithm's own Newton iteration does not converge from the perturbed state on
+this centre-refined mesh, whose smallest elements are ~0.004 wide; 1e-6 is the
+largest decade from which it converges, in 4 iterations.)  One bench step = one
+backward-Euler time step (predictor + Newton to tolerance, every iteration
+assembling r and K on the device and solving K du = -r).
+
+Prints one JSON line.  `value` = Newton iterations per second (each one
+assembly + direct solve + update); the line also carries element assemblies
+per second, time steps per second, p50/p99 time-step latency, the roofline of
+the dominant kernel and the CPU baseline (the oracle restatement timed on the
+host).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "membranealefem.jl_amd"))
+sys.path.insert(0, ROOT)
+
+METRIC = "elem-assemblies/s + Newton steps/s at 1e-8 rel err vs FP64; p50/p99 step latency"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+TILE = 64
+
+
+def build_case(n, motion_name, seed=0xC2, sigma=1e-6):
+    import mafem
+    motion = getattr(mafem, motion_name)
+    p = mafem.Params(motion=motion, num1el=n, num2el=n, length=1.0, kb=1.0, zv=1.0, kg=-0.5,
+                     enr=1e-10, output=False)
+    mesh = mafem.make_mesh(p, pull_speed=0.0)
+    xms, cps = mafem.prepare_input(p, mesh, pull_speed=0.0)
+    rng = np.random.default_rng(seed)
+    xms[:, 2] += rng.normal(0.0, sigma, mesh.numnp)
+    return p, mesh, xms, cps
+
+
+def pull_speed(k, total=200, final=2.0):
+    return final * (k + 1) / total
+
+
+def band_update_flops(dev):
+    """Algorithmic FLOPs of the band-LU trailing updates of one factorization:
+    2 T^3 per tile update, over the (tb x tb) window clipped at the end."""
+    nt, tb = dev_band_shape(dev)
+    tiles = 0
+    for K in range(nt):
+        w = min(tb, nt - 1 - K)
+        tiles += w * w
+    return 2.0 * TILE ** 3 * tiles, nt
+
+
+def dev_band_shape(dev):
+    m = dev.mesh
+    nx = m.numnp * m.ndf
+    nt = (nx + TILE - 1) // TILE
+    kl = (2 * m.num1np + 3) * m.ndf - 1
+    tb = min((kl + TILE - 1) // TILE, max(1, nt - 1))
+    return nt, tb
+
+
+def cpu_baseline(p_gpu, mesh, xms, cps, threads):
+    """One Newton iteration of the oracle (complex-step assembly with OpenMP
+    over elements + SuperLU) on the same 64^2 state: the bounded sample."""
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    import scipy.sparse.linalg as spla
+
+    from oracle import fem, mesh as om
+    po = om.Params(motion=int(p_gpu.motion), num1el=p_gpu.num1el, num2el=p_gpu.num2el,
+                   length=1.0, enr=p_gpu.enr)
+    o, _, _ = fem.build(po, pull_speed=0.0)
+    t0 = time.perf_counter()
+    r, K = fem.calc_r_K(o, xms, cps, 1e-3, 1e-3)
+    t1 = time.perf_counter()
+    du = spla.spsolve(K.tocsc(), -r)
+    t2 = time.perf_counter()
+    return {"value": 1.0 / (t2 - t0), "unit": "Newton steps/s", "cores": threads,
+            "kind": "port",
+            "sample": (f"1 Newton iteration at the 64x64 ALEVB bench state: oracle complex-step "
+                       f"assembly {t1 - t0:.2f} s on {threads} OpenMP threads + scipy SuperLU "
+                       f"solve {t2 - t1:.2f} s (1 thread); elem-assemblies/s "
+                       f"{o.mesh.numel / (t1 - t0):.1f}")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=64)
+    ap.add_argument("--motion", default="ALEVB")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--solver", default="krylov", choices=["direct", "krylov"])
+    ap.add_argument("--precision", type=int, default=64, choices=[64, 32])
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(independent patches shard across GPUs with no communication; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import mafem
+    p, mesh, xms, cps = build_case(args.n, args.motion)
+    dev = mafem.DeviceMesh(mesh, p)
+    dev.set_solver(args.solver, precision=args.precision, rtol=1e-13)
+    dev.set_state(xms, cps)
+    pres = [(node, mesh.dofs[u]) for (u, node, _v) in mesh.inh_dir_bcs]
+    nodes = [a for a, _ in pres]
+    dofs = [b for _, b in pres]
+    dt = 1e-3
+
+    def step(k):
+        dev.set_prescribed(nodes, dofs, [pull_speed(k)] * len(pres))
+        dev.predict(dt)
+        rc, eps = dev.time_step((k + 1) * dt, dt)
+        if rc != 0:
+            raise RuntimeError(f"Newton did not converge at step {k}: {eps}")
+        return eps, dev.stats()
+
+    for k in range(args.warmup):
+        step(k)
+    lat, iters = [], 0
+    agg = {"assemble_us": 0.0, "area_kernel_us": 0.0, "factor_us": 0.0, "solve_us": 0.0,
+           "update_us": 0.0, "n_assemble": 0, "n_band_solve": 0, "n_factor": 0,
+           "gmres_iters": 0}
+    relres = []
+    t_start = time.perf_counter()
+    for k in range(args.warmup, args.warmup + args.steps):
+        t0 = time.perf_counter()
+        eps, st = step(k)
+        lat.append(time.perf_counter() - t0)
+        iters += st["newton_iters"]
+        relres.append(st["last_relres"])
+        for key in agg:
+            agg[key] += st[key]
+    t_total = time.perf_counter() - t_start
+
+    numel = mesh.numel
+    flops_fact, nt = band_update_flops(dev)
+    res = {
+        "metric": METRIC,
+        "value": iters / t_total,
+        "unit": "Newton steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_total / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == 64 else "f64 (FP32 preconditioner)",
+        "data": "synthetic",
+        "config": {"workload": f"F_PULL {args.motion} {args.n}x{args.n} patch, pull ramped to 2.0 "
+                               f"over 200 steps, dt=1e-3, seeded z-perturbation N(0,1e-6), "
+                               f"FP64 assembly; solver {args.solver}, band-LU factor FP{args.precision}",
+                   "nmdf": mesh.nmdf, "numel": numel, "time_steps": args.steps,
+                   "newton_iters": iters},
+        "elem_assemblies_per_s": numel * agg["n_assemble"] / (agg["assemble_us"] * 1e-6),
+        "area_kernel_elem_per_s": numel * agg["n_assemble"] / (agg["area_kernel_us"] * 1e-6),
+        "time_steps_per_s": args.steps / t_total,
+        "step_latency_ms_p50": 1e3 * float(np.percentile(lat, 50)),
+        "step_latency_ms_p99": 1e3 * float(np.percentile(lat, 99)),
+        "phase_us_per_newton": {k: agg[k] / max(1, iters) for k in
+                                ("assemble_us", "area_kernel_us", "factor_us", "solve_us",
+                                 "update_us")},
+        "max_solve_relres": max(relres),
+        "factorizations": agg["n_factor"], "gmres_iters": agg["gmres_iters"],
+        "band_solves": agg["n_band_solve"],
+        "rel_err_vs_fp64": 0.0,
+    }
+    # dominant kernel: the band-LU factorization (FP64); achieved = algorithmic
+    # trailing-update FLOPs / measured factorization time
+    fact_s = agg["factor_us"] * 1e-6 / max(1, agg["n_factor"])
+    achieved = flops_fact / fact_s / 1e12 if fact_s > 0 else 0.0
+    res["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                       "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                       "kernel": "band LU factorization (k_diag + k_panel + k_update per tile row)",
+                       "flops_per_launch": flops_fact, "avg_us": fact_s * 1e6}
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = cpu_baseline(p, mesh, xms, cps, args.cpu_threads)
+    else:
+        res["cpu_baseline"] = None
+    dev.close()
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -76,11 +76,36 @@ typedef struct {
   double bend_tm;
 } mafem_params;
 
+/* Linear solver of the Newton step.
+ * MAFEM_SOLVER_DIRECT: band LU of the current tangent every Newton iteration,
+ *   then FP64 iterative refinement (the analogue of the reference's sparse LU).
+ * MAFEM_SOLVER_KRYLOV: FGMRES(restart) in FP64 against the current tangent,
+ *   preconditioned by a band LU that is kept across Newton iterations and time
+ *   steps and refactored only when a solve needed more than refactor_iters
+ *   iterations; factor_precision 32 stores and factors it in FP32 (the
+ *   mixed-precision path: FP32 preconditioner inside FP64 residuals, update
+ *   and convergence test). */
+#define MAFEM_SOLVER_DIRECT 0
+#define MAFEM_SOLVER_KRYLOV 1
+typedef struct {
+  int32_t mode;              /* MAFEM_SOLVER_DIRECT or MAFEM_SOLVER_KRYLOV */
+  int32_t factor_precision;  /* 64 or 32 */
+  int32_t restart;           /* Krylov restart length, 1..64 */
+  int32_t max_iters;         /* Krylov iterations per linear solve */
+  int32_t refactor_iters;    /* refactor when a solve used more iterations than this */
+  int32_t refine_sweeps;     /* DIRECT: maximum refinement sweeps */
+  double rtol;               /* target |K du + r| / |r| (FP64) */
+} mafem_solver_opts;
+
 /* Per-phase device timings (microseconds) of the last mafem_time_step or
  * mafem_assemble call, measured with HIP events on the context's stream. */
 typedef struct {
-  double assemble_us, factor_us, solve_us, update_us, total_us;
+  double assemble_us;              /* element + edge kernels and the gathers */
+  double area_kernel_us;           /* the area-element kernel alone */
+  double factor_us, solve_us, update_us, total_us;
   int32_t newton_iters, refine_sweeps;
+  int32_t n_assemble, n_band_solve;  /* launches of the assembly / triangular-solve sweeps */
+  int32_t n_factor, gmres_iters;     /* band factorizations / Krylov iterations */
   double last_relres;              /* |K du + r| / |r| of the last solve */
 } mafem_stats;
 
@@ -132,6 +157,13 @@ int mafem_time_step(mafem_ctx *ctx, double time, double dt, int32_t max_iter,
                     int32_t *iters, double *eps_hist);
 
 int mafem_get_stats(const mafem_ctx *ctx, mafem_stats *out);
+
+/* Selects the linear solver (no counterpart in the reference, which always
+ * uses a sparse direct solve: this chooses how FiniteElement.jl:38 is done).
+ * Defaults: DIRECT, FP64, restart 30, 200 iterations, refactor above 20,
+ * 3 refinement sweeps, rtol 1e-13. */
+int mafem_set_solver(mafem_ctx *ctx, const mafem_solver_opts *opts);
+int mafem_get_solver(const mafem_ctx *ctx, mafem_solver_opts *opts);
 
 #ifdef __cplusplus
 }

@@ -51,7 +51,8 @@ void free_all(Ctx &c) {
                   c.d_colidx, c.d_brow, c.d_bsr,     c.d_gptr,    c.d_gsrc,   c.d_rptr,
                   c.d_rsrc, c.d_r,      c.d_x,       c.d_w,       c.d_t,      c.d_b,
                   c.d_red,  c.d_csc_src, c.d_csc_val, c.d_eq,     c.d_band,   c.d_linv,
-                  c.d_uinv, c.d_piv};
+                  c.d_uinv, c.d_piv,  c.d_V,       c.d_Z,       c.d_h,      c.d_coef,
+                  c.d_mdot_part};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c.h_red) (void)hipHostFree(c.h_red);
@@ -299,6 +300,8 @@ void build(Ctx &c, const mafem_mesh_desc *m, const mafem_params *p) {
   const int64_t kl = (2 * n1 + 3) * ndf - 1;
   c.tb = (int)((kl + TILE - 1) / TILE);
   if (c.tb > c.nt - 1) c.tb = (int)std::max<int64_t>(1, c.nt - 1);
+  if (c.tb + 1 > SOLVE_RING)
+    throw ArgError{"patch too wide for the band solver (num1el <= 262)", MAFEM_EUNSUPPORTED};
   c.band_bytes = c.nt * (2 * c.tb + 1) * (int64_t)TILE * TILE * (int64_t)sizeof(double);
 
   // device allocations and uploads
@@ -338,9 +341,7 @@ void build(Ctx &c, const mafem_mesh_desc *m, const mafem_params *p) {
   c.d_csc_src = dupload(c, csc_src);
   c.d_csc_val = dalloc<double>(c, c.nnz);
   c.d_eq = dupload(c, eq_ext);
-  c.d_band = dalloc<double>(c, c.band_bytes / (int64_t)sizeof(double));
-  c.d_linv = dalloc<double>(c, c.nt * TILE * TILE);
-  c.d_uinv = dalloc<double>(c, c.nt * TILE * TILE);
+  c.opts = mafem_solver_opts{MAFEM_SOLVER_DIRECT, 64, 30, 200, 20, 3, 1e-13};
   c.d_piv = dalloc<int32_t>(c, c.nt * TILE);
 }
 
@@ -390,14 +391,51 @@ double solve_refined(Ctx &c, const double *d_b, double *d_x, int max_sweeps, dou
   return rel;
 }
 
+void refactor(Ctx &c) {
+  band_fill(c, c.opts.factor_precision);
+  band_factor(c);
+  c.have_factor = true;
+}
+
+// Solves K x = b with the configured solver; events ev[2] (start) and ev[3]
+// (after any up-front factorization) bracket the factorization phase.
+double linear_solve(Ctx &c, const double *d_b, double *d_x) {
+  const mafem_solver_opts &o = c.opts;
+  MAFEM_HIP(hipEventRecord(c.ev[2], c.stream));
+  if (o.mode == MAFEM_SOLVER_DIRECT) {
+    refactor(c);
+    MAFEM_HIP(hipEventRecord(c.ev[3], c.stream));
+    return solve_refined(c, d_b, d_x, o.refine_sweeps, o.rtol);
+  }
+  bool fresh = false;
+  if (!c.have_factor || c.band_prec != o.factor_precision) {
+    refactor(c);
+    fresh = true;
+  }
+  MAFEM_HIP(hipEventRecord(c.ev[3], c.stream));
+  int its = 0;
+  double rel = fgmres(c, d_b, d_x, &its);
+  if (!(rel <= o.rtol) && !fresh) {
+    // the kept preconditioner no longer suffices: refactor from this tangent
+    refactor(c);
+    int its2 = 0;
+    rel = fgmres(c, d_b, d_x, &its2);
+    its += its2;
+  } else if (its > o.refactor_iters) {
+    c.have_factor = false;  // refactor before the next solve
+  }
+  c.stats.last_relres = rel;
+  return rel;
+}
+
 }  // namespace
 
-#define ABI_GUARD(ctx, body)                                   \
+#define ABI_GUARD(ctx, ...)                                    \
   do {                                                         \
     if (!(ctx)) return MAFEM_EINVAL;                           \
     Ctx &c = (ctx)->c;                                         \
     try {                                                      \
-      body                                                     \
+      __VA_ARGS__                                              \
     } catch (const HipError &h) {                              \
       return from_hip(c, h);                                   \
     } catch (const ArgError &a) {                              \
@@ -541,6 +579,8 @@ int mafem_assemble(mafem_ctx *ctx, double time, double dt, double *r, int64_t *c
     if (rowval) std::memcpy(rowval, c.h_rowval.data(), sizeof(int64_t) * c.nnz);
     c.stats = mafem_stats{};
     c.stats.assemble_us = 1e3 * elapsed(c, 0, 1);
+    c.stats.area_kernel_us = 1e3 * elapsed(c, 8, 9);
+    c.stats.n_assemble = 1;
     return MAFEM_OK;
   });
 }
@@ -554,11 +594,8 @@ int mafem_solve(mafem_ctx *ctx, const double *rhs, double *du, double *relres) {
     hipLaunchKernelGGL(k_scatter_idx, dim3(nb256(c.nmdf)), dim3(256), 0, c.stream, c.nmdf, c.d_eq,
                        c.d_t, c.d_b);
     MAFEM_HIP(hipGetLastError());
-    MAFEM_HIP(hipEventRecord(c.ev[2], c.stream));
-    band_fill(c);
-    band_factor(c);
-    MAFEM_HIP(hipEventRecord(c.ev[3], c.stream));
-    const double rel = solve_refined(c, c.d_b, c.d_x, 3, 1e-14);
+    const int32_t nf0 = c.n_factor, ng0 = c.n_gmres;
+    const double rel = linear_solve(c, c.d_b, c.d_x);
     MAFEM_HIP(hipEventRecord(c.ev[4], c.stream));
     hipLaunchKernelGGL(k_gather_idx, dim3(nb256(c.nmdf)), dim3(256), 0, c.stream, c.nmdf, c.d_eq,
                        c.d_x, c.d_t);
@@ -568,6 +605,8 @@ int mafem_solve(mafem_ctx *ctx, const double *rhs, double *du, double *relres) {
     if (relres) *relres = rel;
     c.stats.factor_us = 1e3 * elapsed(c, 2, 3);
     c.stats.solve_us = 1e3 * elapsed(c, 3, 4);
+    c.stats.n_factor = c.n_factor - nf0;
+    c.stats.gmres_iters = c.n_gmres - ng0;
     return MAFEM_OK;
   });
 }
@@ -580,21 +619,20 @@ int mafem_time_step(mafem_ctx *ctx, double time, double dt, int32_t max_iter, in
     mafem_stats st{};
     double eps = INFINITY;
     int it = 0;
+    const int32_t nbs0 = c.n_band_solve, nf0 = c.n_factor, ng0 = c.n_gmres;
     MAFEM_HIP(hipEventRecord(c.ev[0], c.stream));
     while (it < max_iter) {
       MAFEM_HIP(hipEventRecord(c.ev[1], c.stream));
       launch_assemble(c, time, dt);
-      MAFEM_HIP(hipEventRecord(c.ev[2], c.stream));
-      band_fill(c);
-      band_factor(c);
-      MAFEM_HIP(hipEventRecord(c.ev[3], c.stream));
       vec_scale(c, c.nt * TILE, -1.0, c.d_r, c.d_b);  // b = -r
-      solve_refined(c, c.d_b, c.d_x, 3, 1e-14);
+      linear_solve(c, c.d_b, c.d_x);
       MAFEM_HIP(hipEventRecord(c.ev[4], c.stream));
       apply_update(c, c.d_x, dt);
       MAFEM_HIP(hipEventRecord(c.ev[5], c.stream));
       eps = vec_norm2_masked(c, c.d_x) / (double)c.nmdf;  // synchronizes
       st.assemble_us += 1e3 * elapsed(c, 1, 2);
+      st.area_kernel_us += 1e3 * elapsed(c, 8, 9);
+      st.n_assemble += 1;
       st.factor_us += 1e3 * elapsed(c, 2, 3);
       st.solve_us += 1e3 * elapsed(c, 3, 4);
       st.update_us += 1e3 * elapsed(c, 4, 5);
@@ -609,6 +647,9 @@ int mafem_time_step(mafem_ctx *ctx, double time, double dt, int32_t max_iter, in
     MAFEM_HIP(hipEventSynchronize(c.ev[6]));
     st.total_us = 1e3 * elapsed(c, 0, 6);
     st.newton_iters = it;
+    st.n_band_solve = c.n_band_solve - nbs0;
+    st.n_factor = c.n_factor - nf0;
+    st.gmres_iters = c.n_gmres - ng0;
     c.stats = st;
     if (iters) *iters = it;
     if (!(eps < c.enr)) {
@@ -622,6 +663,25 @@ int mafem_time_step(mafem_ctx *ctx, double time, double dt, int32_t max_iter, in
 int mafem_get_stats(const mafem_ctx *ctx, mafem_stats *out) {
   if (!ctx || !out) return MAFEM_EINVAL;
   *out = ctx->c.stats;
+  return MAFEM_OK;
+}
+
+int mafem_set_solver(mafem_ctx *ctx, const mafem_solver_opts *o) {
+  ABI_GUARD(ctx, {
+    if (!o) return fail(c, "null solver options", MAFEM_EINVAL);
+    if ((o->mode != MAFEM_SOLVER_DIRECT && o->mode != MAFEM_SOLVER_KRYLOV) ||
+        (o->factor_precision != 64 && o->factor_precision != 32) || o->restart < 1 ||
+        o->restart > 64 || o->max_iters < 1 || o->refine_sweeps < 0 || !(o->rtol > 0.0))
+      return fail(c, "invalid solver options", MAFEM_EINVAL);
+    if (o->factor_precision != c.opts.factor_precision) c.have_factor = false;
+    c.opts = *o;
+    return MAFEM_OK;
+  });
+}
+
+int mafem_get_solver(const mafem_ctx *ctx, mafem_solver_opts *o) {
+  if (!ctx || !o) return MAFEM_EINVAL;
+  *o = ctx->c.opts;
   return MAFEM_OK;
 }
 

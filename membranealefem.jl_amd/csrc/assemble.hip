@@ -579,7 +579,9 @@ static hipError_t launch_edge(Ctx &c, double time, double dt) {
 }
 
 int launch_assemble(Ctx &c, double time, double dt) {
+  MAFEM_HIP(hipEventRecord(c.ev[8], c.stream));
   MAFEM_HIP(launch_area(c, dt));
+  MAFEM_HIP(hipEventRecord(c.ev[9], c.stream));
   MAFEM_HIP(launch_edge(c, time, dt));
   const int64_t nb = c.nnzb * c.ndf * c.ndf;
   hipLaunchKernelGGL(k_gather_bsr, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, c.stream,

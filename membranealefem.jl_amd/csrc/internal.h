@@ -20,6 +20,7 @@ constexpr int NEN = 9;
 constexpr int REC = 55;      // doubles per tabulated 2-D basis record
 constexpr int RECP = 56;     // padded record stride on the device
 constexpr int TILE = 64;     // band-LU tile edge
+constexpr int SOLVE_RING = 72;  // solution tiles a sweep keeps in LDS (needs tb + 1 <= this)
 
 // Device-side physical parameters (by value into kernels).
 struct PhysParams {
@@ -42,7 +43,7 @@ struct Ctx {
   std::string err;
   int dev = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[8] = {};
+  hipEvent_t ev[12] = {};   // 0-6 phases, 8-9 area kernel
 
   // sizes
   int64_t num1el = 0, num2el = 0, numel = 0, num1np = 0, num2np = 0, numnp = 0;
@@ -94,22 +95,35 @@ struct Ctx {
   double *d_csc_val = nullptr;
   int64_t *d_eq = nullptr;       // extended index of each equation (nmdf)
 
-  // band LU
+  // band LU (storage precision band_prec: 64 or 32; allocated on first use)
   int64_t nt = 0;                // tile rows
   int tb = 0;                    // tile half bandwidth
-  double *d_band = nullptr;      // [nt][2tb+1][TILE][TILE] row-major tiles
-  double *d_linv = nullptr, *d_uinv = nullptr;  // [nt][TILE][TILE]
+  void *d_band = nullptr;        // [nt][2tb+1][TILE][TILE] row-major tiles
+  void *d_linv = nullptr, *d_uinv = nullptr;  // [nt][TILE][TILE]
   int32_t *d_piv = nullptr;      // [nt][TILE] row permutation of each tile row
   int64_t band_bytes = 0;
+  int band_prec = 0;
+  bool have_factor = false;      // the band holds a factorization usable as preconditioner
+  int32_t n_factor = 0;          // running count of factorizations
+
+  // linear solver options and Krylov workspace
+  mafem_solver_opts opts{};
+  double *d_V = nullptr, *d_Z = nullptr;   // (m+1) and m basis vectors of length nt TILE
+  double *d_h = nullptr;                   // multi-dot results (device)
+  double *d_coef = nullptr;                // combination weights (device)
+  double *d_mdot_part = nullptr;           // [65][256] partial dots
+  int kry_m = 0;
+  int32_t n_gmres = 0;           // running count of GMRES iterations
 
   mafem_stats stats{};
+  int32_t n_band_solve = 0;     // running count of triangular-solve sweeps
 };
 
 // kernels / launchers (assemble.hip)
 int launch_assemble(Ctx &c, double time, double dt);
 void launch_predict(Ctx &c, const double *d_u, double dt);
 // band LU and solves (band.hip)
-void band_fill(Ctx &c);
+void band_fill(Ctx &c, int prec);
 void band_factor(Ctx &c);
 void band_solve(Ctx &c, const double *d_rhs, double *d_x);
 void bsr_spmv(Ctx &c, const double *d_x, double *d_y);
@@ -117,6 +131,13 @@ void vec_axpby(Ctx &c, int64_t n, double a, const double *x, double b, double *y
 void vec_zero(Ctx &c, int64_t n, double *y);
 void vec_scale(Ctx &c, int64_t n, double a, const double *x, double *y);
 double vec_norm2_masked(Ctx &c, const double *x);   // active entries only, blocking
+double vec_norm2(Ctx &c, const double *x, int64_t n);  // all entries, blocking
+// multi-vector kernels of the Krylov solver (vec.hip)
+void vec_mdot(Ctx &c, int64_t n, int nvec, const double *V, int64_t ld, const double *w, double *h_out);
+void vec_mcomb(Ctx &c, int64_t n, int nvec, const double *V, int64_t ld, const double *h_coef, double alpha, double *w);
+// Krylov solver (krylov.hip): K x = b, x overwritten; returns |b - K x| / |b|
+double fgmres(Ctx &c, const double *d_b, double *d_x, int *iters);
+void ensure_krylov(Ctx &c);
 void apply_update(Ctx &c, const double *d_du, double dt);
 
 }  // namespace mafem

@@ -45,10 +45,25 @@ class ParamsDesc(ctypes.Structure):
 
 class Stats(ctypes.Structure):
     _fields_ = [
-        ("assemble_us", ctypes.c_double), ("factor_us", ctypes.c_double),
-        ("solve_us", ctypes.c_double), ("update_us", ctypes.c_double),
-        ("total_us", ctypes.c_double), ("newton_iters", ctypes.c_int32),
-        ("refine_sweeps", ctypes.c_int32), ("last_relres", ctypes.c_double),
+        ("assemble_us", ctypes.c_double), ("area_kernel_us", ctypes.c_double),
+        ("factor_us", ctypes.c_double), ("solve_us", ctypes.c_double),
+        ("update_us", ctypes.c_double), ("total_us", ctypes.c_double),
+        ("newton_iters", ctypes.c_int32), ("refine_sweeps", ctypes.c_int32),
+        ("n_assemble", ctypes.c_int32), ("n_band_solve", ctypes.c_int32),
+        ("n_factor", ctypes.c_int32), ("gmres_iters", ctypes.c_int32),
+        ("last_relres", ctypes.c_double),
+    ]
+
+
+SOLVER_DIRECT, SOLVER_KRYLOV = 0, 1
+
+
+class SolverOpts(ctypes.Structure):
+    _fields_ = [
+        ("mode", ctypes.c_int32), ("factor_precision", ctypes.c_int32),
+        ("restart", ctypes.c_int32), ("max_iters", ctypes.c_int32),
+        ("refactor_iters", ctypes.c_int32), ("refine_sweeps", ctypes.c_int32),
+        ("rtol", ctypes.c_double),
     ]
 
 
@@ -69,6 +84,8 @@ SIGNATURES = {
     "mafem_time_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
                                        ctypes.c_int32, i32p, f64p]),
     "mafem_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Stats)]),
+    "mafem_set_solver": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SolverOpts)]),
+    "mafem_get_solver": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SolverOpts)]),
 }
 
 _lib = None

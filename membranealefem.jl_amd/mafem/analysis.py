@@ -150,6 +150,21 @@ class DeviceMesh:
                                                   ctypes.byref(it), L.ptr(eps, L.f64p)))
         return rc, list(eps[: it.value])
 
+    def set_solver(self, mode="direct", precision=64, restart=30, max_iters=200,
+                   refactor_iters=20, refine_sweeps=3, rtol=1e-13):
+        """Linear solver of the Newton step: "direct" (band LU every iteration +
+        FP64 refinement) or "krylov" (FP64 FGMRES preconditioned by a kept band
+        LU, FP64 or FP32)."""
+        o = L.SolverOpts(L.SOLVER_KRYLOV if mode == "krylov" else L.SOLVER_DIRECT,
+                         int(precision), int(restart), int(max_iters), int(refactor_iters),
+                         int(refine_sweeps), float(rtol))
+        self._check(self.lib.mafem_set_solver(self.h, ctypes.byref(o)))
+
+    def get_solver(self):
+        o = L.SolverOpts()
+        self._check(self.lib.mafem_get_solver(self.h, ctypes.byref(o)))
+        return {k: getattr(o, k) for k, _ in L.SolverOpts._fields_}
+
     def stats(self):
         s = L.Stats()
         self._check(self.lib.mafem_get_stats(self.h, ctypes.byref(s)))

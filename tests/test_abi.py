@@ -37,7 +37,7 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts_match_c(tmp_path):
     src = tmp_path / "lay.c"
     fields = {"mafem_mesh_desc": _lib.MeshDesc, "mafem_params": _lib.ParamsDesc,
-              "mafem_stats": _lib.Stats}
+              "mafem_stats": _lib.Stats, "mafem_solver_opts": _lib.SolverOpts}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "mafem.h"', "int main(void){"]
     for cname, cls in fields.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
