@@ -52,7 +52,7 @@ void free_all(Ctx &c) {
                   c.d_rsrc, c.d_r,      c.d_x,       c.d_w,       c.d_t,      c.d_b,
                   c.d_red,  c.d_csc_src, c.d_csc_val, c.d_eq,     c.d_band,   c.d_linv,
                   c.d_uinv, c.d_piv,  c.d_V,       c.d_Z,       c.d_h,      c.d_coef,
-                  c.d_mdot_part};
+                  c.d_mdot_part, c.d_t2};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c.h_red) (void)hipHostFree(c.h_red);
@@ -334,7 +334,8 @@ void build(Ctx &c, const mafem_mesh_desc *m, const mafem_params *p) {
   c.d_w = dalloc<double>(c, nv);
   c.d_t = dalloc<double>(c, nv);
   c.d_b = dalloc<double>(c, nv);
-  for (double *v : {c.d_r, c.d_x, c.d_w, c.d_t, c.d_b})
+  c.d_t2 = dalloc<double>(c, nv);
+  for (double *v : {c.d_r, c.d_x, c.d_w, c.d_t, c.d_b, c.d_t2})
     MAFEM_HIP(hipMemset(v, 0, sizeof(double) * nv));
   c.d_red = dalloc<double>(c, 512);
   MAFEM_HIP(hipHostMalloc((void **)&c.h_red, sizeof(double) * 16, hipHostMallocDefault));

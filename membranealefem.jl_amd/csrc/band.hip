@@ -6,13 +6,15 @@
 // half bandwidth kl = (2 num1np + 3) ndf - 1 and LU without pivoting across
 // tile rows keeps the fill inside that band.  Partial pivoting is applied
 // inside each diagonal tile.  Per tile step K:
-//   k_diag   : P_K A_KK = L_KK U_KK in one workgroup (tile in LDS), then
-//              explicit inverses of L_KK and U_KK;
-//   k_panel  : U_KJ = L_KK^-1 P_K A_KJ, L_IK = A_IK U_KK^-1, and P_K applied to
-//              the L tiles already stored in tile row K;
+//   k_diag   : P_K A_KK = L_KK U_KK in one wavefront (rows in registers);
+//   k_panel  : U_KJ = L_KK^-1 P_K A_KJ, L_IK = A_IK U_KK^-1 (triangular
+//              solves, one wavefront per tile) and P_K applied to the L tiles
+//              already stored in tile row K;
 //   k_update : A_IJ -= L_IK U_KJ over the (tb x tb) trailing window.
-// The triangular sweeps run in one workgroup each (the tile rows form a
-// dependency chain) and accumulate in FP64 whatever the storage precision.
+// Afterwards a fully parallel pass inverts the diagonal tiles and folds the
+// inverses into the off-diagonal tiles.  The triangular sweeps run in one
+// workgroup each (the tile rows form a dependency chain) and accumulate in
+// FP64 whatever the storage precision.
 // This replaces `K_gl \ r_gl` (src/analysis/FiniteElement.jl:38, UMFPACK).
 #include <algorithm>
 
@@ -73,102 +75,105 @@ __device__ __forceinline__ void wave_argmax(S &v, int &idx) {
   }
 }
 
-// Makes this wave's LDS writes visible to its other lanes before later reads.
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+// value of x held by lane l (l wave-uniform), delivered as a scalar
+__device__ __forceinline__ double lane_bcast(double x, int l) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ float lane_bcast(float x, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
 }
 
-// Factors the 64x64 diagonal tile with partial pivoting (P A = L U) and forms
-// L^-1 and U^-1.  The tile is held column-major in LDS (sT[col][row], padded)
-// so that lane = row accesses are contiguous.  Wave 0 picks the pivot, swaps
-// rows and forms the multipliers; all four waves share the rank-1 update by
-// columns.  Then wave 0 inverts L and wave 1 inverts U, both in place (the
-// strictly lower and the upper parts are disjoint).
+// LU with partial pivoting of the 64x64 diagonal tile in one wavefront: lane
+// i holds row i in registers.  Rows are not moved during the elimination: at
+// step j the largest |a_ij| among the rows not yet chosen is the pivot, its
+// row is broadcast element by element with readlane, and every other
+// unchosen row is eliminated.  The row chosen at step j is stored as row j of
+// the factored tile (P A = L U) and piv[K][j] records its original index.
 template <class S>
-__global__ __launch_bounds__(256) void k_diag(int64_t K, int tb, S *__restrict__ band,
-                                              S *__restrict__ linv, S *__restrict__ uinv,
-                                              int32_t *__restrict__ piv) {
-  __shared__ S sT[T][T + 1];  // sT[col][row]
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+__global__ __launch_bounds__(64) void k_diag(int64_t K, int tb, S *__restrict__ band,
+                                             int32_t *__restrict__ piv) {
+  const int lane = threadIdx.x;
   S *A = tile_ptr(band, tb, K, K);
-  for (int k = tid; k < T * T; k += 256) sT[k % T][k / T] = A[k];
-  int perm = lane;
-  __syncthreads();
+  S r[T];
+#pragma unroll
+  for (int k = 0; k < T; ++k) r[k] = A[lane * T + k];
+  int pos = -1;
+#pragma unroll
   for (int j = 0; j < T; ++j) {
-    if (wv == 0) {
-      S v = (lane >= j) ? fabs(sT[j][lane]) : S(-1);
-      int p = lane;
-      wave_argmax(v, p);
-      if (p != j) {  // swap rows j and p: lane k swaps column k
-        const S a = sT[lane][j], b = sT[lane][p];
-        sT[lane][j] = b;
-        sT[lane][p] = a;
-        const int pj = __shfl(perm, j, 64), pp = __shfl(perm, p, 64);
-        if (lane == j) perm = pp;
-        if (lane == p) perm = pj;
-      }
-      wave_lds_sync();
-      if (lane > j) sT[j][lane] = sT[j][lane] / sT[j][j];
+    S v = (pos < 0) ? fabs(r[j]) : S(-1);
+    int p = lane;
+    wave_argmax(v, p);
+    p = __builtin_amdgcn_readfirstlane(p);
+    if (lane == p) pos = j;
+    const bool elim = pos < 0;
+    const S pj = lane_bcast(r[j], p);
+    const S l = r[j] / pj;
+    if (elim) r[j] = l;
+#pragma unroll
+    for (int k = j + 1; k < T; ++k) {
+      const S pk = lane_bcast(r[k], p);
+      if (elim) r[k] = fma(-l, pk, r[k]);
     }
-    __syncthreads();
-    if (lane > j) {
-      const S l = sT[j][lane];
-#pragma unroll 4
-      for (int k = j + 1 + wv; k < T; k += 4) sT[k][lane] = fma(-l, sT[k][j], sT[k][lane]);
-    }
-    __syncthreads();
   }
-  for (int k = tid; k < T * T; k += 256) A[k] = sT[k % T][k / T];
-  if (wv == 0) piv[K * T + lane] = perm;
-  __syncthreads();  // the tile is copied out before the in-place inversions
+#pragma unroll
+  for (int k = 0; k < T; ++k) A[pos * T + k] = r[k];
+  piv[K * T + pos] = lane;
+}
 
-  if (wv == 0) {
-    // V = L^-1 (unit lower), columns j descending: V_ij = -Σ_{k=j+1}^{i} V_ik L_kj
-    const int i = lane;
-    for (int j = T - 2; j >= 0; --j) {
-      S s0 = 0, s1 = 0;
-      if (i > j) {
-        int k = j + 1;
-        for (; k + 1 < i; k += 2) {
-          s0 = fma(sT[k][i], sT[j][k], s0);
-          s1 = fma(sT[k + 1][i], sT[j][k + 1], s1);
-        }
-        if (k < i) s0 = fma(sT[k][i], sT[j][k], s0);
-        s0 += sT[j][i];  // k = i: V_ii = 1 times L_ij
-      }
-      wave_lds_sync();
-      if (i > j) sT[j][i] = -(s0 + s1);
-      wave_lds_sync();
+// Panel of step K, one wavefront per tile.  kind 0: U_KJ = L_KK^-1 P_K A_KJ
+// (lane = column, forward substitution); kind 1: L_IK = A_IK U_KK^-1 (lane =
+// row, substitution against U); kind 2: P_K applied to the L tiles already
+// stored left of the diagonal in tile row K.  The factored diagonal tile is
+// read with wave-uniform addresses.
+template <class S>
+__global__ __launch_bounds__(64) void k_panel(int64_t K, int64_t nt, int tb, S *__restrict__ band,
+                                              const int32_t *__restrict__ piv) {
+  const int lane = threadIdx.x;
+  const int kind = blockIdx.x / tb, off = blockIdx.x % tb;
+  const int32_t *pv = piv + K * T;
+  const S *D = tile_ptr(band, tb, K, K);
+  S x[T];
+  if (kind == 0) {
+    const int64_t J = K + 1 + off;
+    if (J >= nt) return;
+    S *A = tile_ptr(band, tb, K, J);
+#pragma unroll
+    for (int i = 0; i < T; ++i) x[i] = A[(int64_t)pv[i] * T + lane];
+#pragma unroll
+    for (int i = 1; i < T; ++i) {
+      S s = x[i];
+#pragma unroll
+      for (int k = 0; k < i; ++k) s = fma(-D[i * T + k], x[k], s);
+      x[i] = s;
     }
-  } else if (wv == 1) {
-    // W = U^-1 (upper), columns j ascending: W_ij = -Σ_{k=i}^{j-1} W_ik U_kj / U_jj
-    const int i = lane;
+#pragma unroll
+    for (int i = 0; i < T; ++i) A[i * T + lane] = x[i];
+  } else if (kind == 1) {
+    const int64_t I = K + 1 + off;
+    if (I >= nt) return;
+    S *A = tile_ptr(band, tb, I, K);
+#pragma unroll
+    for (int k = 0; k < T; ++k) x[k] = A[lane * T + k];
+#pragma unroll
     for (int j = 0; j < T; ++j) {
-      const S ujj = sT[j][j];
-      S s0 = 0, s1 = 0;
-      if (i < j) {
-        int k = i;
-        for (; k + 1 < j; k += 2) {
-          s0 = fma(sT[k][i], sT[j][k], s0);
-          s1 = fma(sT[k + 1][i], sT[j][k + 1], s1);
-        }
-        if (k < j) s0 = fma(sT[k][i], sT[j][k], s0);
-      }
-      wave_lds_sync();
-      if (i < j) sT[j][i] = -(s0 + s1) / ujj;
-      if (i == j) sT[j][j] = S(1) / ujj;
-      wave_lds_sync();
+      S s = x[j];
+#pragma unroll
+      for (int k = 0; k < j; ++k) s = fma(-x[k], D[k * T + j], s);
+      x[j] = s / D[j * T + j];
     }
-  }
-  __syncthreads();
-  S *Li = linv + K * (int64_t)T * T;
-  S *Ui = uinv + K * (int64_t)T * T;
-  for (int k = tid; k < T * T; k += 256) {
-    const int i = k / T, jj = k % T;
-    Li[k] = (jj < i) ? sT[jj][i] : (jj == i ? S(1) : S(0));
-    Ui[k] = (jj >= i) ? sT[jj][i] : S(0);
+#pragma unroll
+    for (int k = 0; k < T; ++k) A[lane * T + k] = x[k];
+  } else {
+    const int64_t J = K - tb + off;
+    if (J < 0) return;
+    S *A = tile_ptr(band, tb, K, J);
+#pragma unroll
+    for (int i = 0; i < T; ++i) x[i] = A[(int64_t)pv[i] * T + lane];
+#pragma unroll
+    for (int i = 0; i < T; ++i) A[i * T + lane] = x[i];
   }
 }
 
@@ -195,58 +200,6 @@ __device__ __forceinline__ void tile_mm(const S (*sa)[T], const S (*sb)[T], S c[
 }
 
 template <class S>
-__global__ __launch_bounds__(256) void k_panel(int64_t K, int64_t nt, int tb, S *__restrict__ band,
-                                               const S *__restrict__ linv,
-                                               const S *__restrict__ uinv,
-                                               const int32_t *__restrict__ piv) {
-  __shared__ S sa[T][T];
-  __shared__ S sb[T][T];
-  const int kind = blockIdx.x / tb, off = blockIdx.x % tb;
-  const int32_t *pv = piv + K * T;
-  const int i0 = (threadIdx.x / 16) * 4, j0 = (threadIdx.x % 16) * 4;
-  if (kind == 0) {  // U_KJ = L_KK^-1 P_K A_KJ
-    const int64_t J = K + 1 + off;
-    if (J >= nt) return;
-    S *A = tile_ptr(band, tb, K, J);
-    const S *Li = linv + K * (int64_t)T * T;
-    for (int k = threadIdx.x; k < T * T; k += 256) {
-      sa[k / T][k % T] = Li[k];
-      sb[k / T][k % T] = A[(int64_t)pv[k / T] * T + (k % T)];
-    }
-    __syncthreads();
-    S c[4][4];
-    tile_mm<S>(sa, sb, c);
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int b = 0; b < 4; ++b) A[(i0 + a) * T + j0 + b] = c[a][b];
-  } else if (kind == 1) {  // L_IK = A_IK U_KK^-1
-    const int64_t I = K + 1 + off;
-    if (I >= nt) return;
-    S *A = tile_ptr(band, tb, I, K);
-    const S *Ui = uinv + K * (int64_t)T * T;
-    for (int k = threadIdx.x; k < T * T; k += 256) {
-      sa[k / T][k % T] = A[k];
-      sb[k / T][k % T] = Ui[k];
-    }
-    __syncthreads();
-    S c[4][4];
-    tile_mm<S>(sa, sb, c);
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int b = 0; b < 4; ++b) A[(i0 + a) * T + j0 + b] = c[a][b];
-  } else {  // row permutation of the L tiles left of the diagonal in row K
-    const int64_t J = K - tb + off;
-    if (J < 0) return;
-    S *A = tile_ptr(band, tb, K, J);
-    for (int k = threadIdx.x; k < T * T; k += 256) sa[k / T][k % T] = A[k];
-    __syncthreads();
-    for (int k = threadIdx.x; k < T * T; k += 256) A[k] = sa[pv[k / T]][k % T];
-  }
-}
-
-template <class S>
 __global__ __launch_bounds__(256) void k_update(int64_t K, int64_t nt, int tb, S *__restrict__ band) {
   __shared__ S sa[T][T];
   __shared__ S sb[T][T];
@@ -269,12 +222,82 @@ __global__ __launch_bounds__(256) void k_update(int64_t K, int64_t nt, int tb, S
     for (int b = 0; b < 4; ++b) A[(i0 + a) * T + j0 + b] -= c[a][b];
 }
 
+// ------------------------------------------------- post-factorization pass
+//
+// Inverts every factored diagonal tile (one wavefront per tile, lane = column
+// of the inverse) and folds the inverses into the off-diagonal tiles,
+// L~_IK = L_II^-1 L_IK and U~_IJ = U_II^-1 U_IJ, so that each sweep step is a
+// single batch of tile mat-vecs: y_I = L_II^-1 P_I b_I - Σ L~_IK y_K and
+// x_I = U_II^-1 y_I - Σ U~_IJ x_J.  Every tile is independent: fully parallel.
+
+template <class S>
+__global__ __launch_bounds__(64) void k_tile_inv(int tb, const S *__restrict__ band,
+                                                 S *__restrict__ linv, S *__restrict__ uinv) {
+  const int64_t I = blockIdx.x;
+  const int c = threadIdx.x;
+  const S *D = tile_ptr(band, tb, I, I);
+  S x[T];
+  // L^-1 e_c, L unit lower
+#pragma unroll
+  for (int i = 0; i < T; ++i) {
+    S s = (i == c) ? S(1) : S(0);
+#pragma unroll
+    for (int k = 0; k < i; ++k) s = fma(-D[i * T + k], x[k], s);
+    x[i] = s;
+  }
+  S *Li = linv + I * (int64_t)T * T;
+#pragma unroll
+  for (int i = 0; i < T; ++i) Li[i * T + c] = x[i];
+  // U^-1 e_c
+#pragma unroll
+  for (int i = T - 1; i >= 0; --i) {
+    S s = (i == c) ? S(1) : S(0);
+#pragma unroll
+    for (int k = i + 1; k < T; ++k) s = fma(-D[i * T + k], x[k], s);
+    x[i] = s / D[i * T + i];
+  }
+  S *Ui = uinv + I * (int64_t)T * T;
+#pragma unroll
+  for (int i = 0; i < T; ++i) Ui[i * T + c] = x[i];
+}
+
+// blockIdx.x = I (2 tb + 1 slots): slot < tb -> L tile (I, I - tb + slot),
+// slot > tb -> U tile (I, I - tb + slot); the diagonal slot is skipped.
+template <class S>
+__global__ __launch_bounds__(256) void k_fold(int64_t nt, int tb, S *__restrict__ band,
+                                              const S *__restrict__ linv,
+                                              const S *__restrict__ uinv) {
+  __shared__ S sa[T][T];
+  __shared__ S sb[T][T];
+  const int64_t I = blockIdx.x / (2 * tb + 1);
+  const int slot = blockIdx.x % (2 * tb + 1);
+  const int64_t J = I - tb + slot;
+  if (slot == tb || J < 0 || J >= nt) return;
+  const S *Inv = (slot < tb ? linv : uinv) + I * (int64_t)T * T;
+  S *A = tile_ptr(band, tb, I, J);
+  for (int k = threadIdx.x; k < T * T; k += 256) {
+    sa[k / T][k % T] = Inv[k];
+    sb[k / T][k % T] = A[k];
+  }
+  __syncthreads();
+  S c[4][4];
+  tile_mm<S>(sa, sb, c);
+  const int i0 = (threadIdx.x / 16) * 4, j0 = (threadIdx.x % 16) * 4;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) A[(i0 + a) * T + j0 + b] = c[a][b];
+}
+
 // ------------------------------------------------------------ triangular solves
 //
 // The band's tile rows form a dependency chain, so each sweep runs in one
 // 1024-thread workgroup: no inter-workgroup synchronisation, the last tb
-// solution tiles stay in an LDS ring, and each step streams the <= tb band
-// tiles of its row with 16 threads per tile row (4 columns each).
+// solution tiles stay in an LDS ring, and each step streams the <= tb folded
+// tiles of its row with 16 threads per tile row (4 columns each), all loads
+// of a row in flight together.  The diagonal-block products (L_II^-1 P_I b_I
+// before, U_II^-1 y_I between the sweeps) are independent per tile row and
+// run fully parallel.
 
 constexpr int SW = 1024;          // threads of a sweep workgroup
 constexpr int RING = SOLVE_RING;  // LDS ring of solution tiles (>= tb + 1)
@@ -308,31 +331,66 @@ __device__ __forceinline__ double tile_row4(const S *M, const double *v) {
   return fma((double)m.x, x[0], fma((double)m.y, x[1], fma((double)m.z, x[2], (double)m.w * x[3])));
 }
 
+// Sum over the band tiles (I, K), K in [K0, K1), of tile_row4 against the ring;
+// up to 12 (FP64) or 24 (FP32) tile loads per thread are issued before their
+// results are used.
+template <class S>
+__device__ __forceinline__ double band_row_sum(const S *band, int tb, int64_t I, int64_t K0,
+                                               int64_t K1, const double (*ring)[T]) {
+  constexpr int B = sizeof(S) == 4 ? 24 : 12;  // loads in flight within the VGPR budget
+  const int i = threadIdx.x >> 4, q = threadIdx.x & 15;
+  double acc = 0.0;
+  for (int64_t Kb = K0; Kb < K1; Kb += B) {
+    typename Vec4<S>::type m[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u)
+      if (Kb + u < K1)
+        m[u] = *reinterpret_cast<const typename Vec4<S>::type *>(
+            tile_ptr(band, tb, I, Kb + u) + i * T + 4 * q);
+#pragma unroll
+    for (int u = 0; u < B; ++u)
+      if (Kb + u < K1) {
+        const double *x = ring[(Kb + u) % RING] + 4 * q;
+        acc = fma((double)m[u].x, x[0], acc);
+        acc = fma((double)m[u].y, x[1], acc);
+        acc = fma((double)m[u].z, x[2], acc);
+        acc = fma((double)m[u].w, x[3], acc);
+      }
+  }
+  return acc;
+}
+
+// out_I = M_I v_I for every tile row (M = L^-1 with v = P b, or U^-1 with v = y)
+template <class S>
+__global__ __launch_bounds__(256) void k_diag_apply(const S *__restrict__ inv,
+                                                    const int32_t *__restrict__ piv,
+                                                    const double *__restrict__ v,
+                                                    double *__restrict__ out) {
+  __shared__ double sv[T];
+  const int64_t I = blockIdx.x;
+  if (threadIdx.x < T) sv[threadIdx.x] = v[I * T + (piv ? piv[I * T + threadIdx.x] : threadIdx.x)];
+  __syncthreads();
+  const int i = threadIdx.x / 4, q = threadIdx.x % 4;
+  const S *m = inv + I * (int64_t)T * T + i * T + q * 16;
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) s = fma((double)m[k], sv[q * 16 + k], s);
+  s += __shfl_xor(s, 1, 64);
+  s += __shfl_xor(s, 2, 64);
+  if (q == 0) out[I * T + i] = s;
+}
+
+// y_I = c_I - Σ_{K<I} L~_IK y_K  (c precomputed into y)
 template <class S>
 __global__ __launch_bounds__(SW) void k_fwd_sweep(int64_t nt, int tb, const S *__restrict__ band,
-                                                  const S *__restrict__ linv,
-                                                  const int32_t *__restrict__ piv,
-                                                  const double *__restrict__ b,
                                                   double *__restrict__ y) {
   __shared__ double ring[RING][T];
-  __shared__ double st[T];
   const int i = threadIdx.x >> 4, q = threadIdx.x & 15;
   for (int64_t I = 0; I < nt; ++I) {
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
     const int64_t K0 = I - tb < 0 ? 0 : I - tb;
-    int64_t K = K0;
-    for (; K + 3 < I; K += 4) {
-      s0 += tile_row4(tile_ptr(band, tb, I, K), ring[K % RING]);
-      s1 += tile_row4(tile_ptr(band, tb, I, K + 1), ring[(K + 1) % RING]);
-      s2 += tile_row4(tile_ptr(band, tb, I, K + 2), ring[(K + 2) % RING]);
-      s3 += tile_row4(tile_ptr(band, tb, I, K + 3), ring[(K + 3) % RING]);
-    }
-    for (; K < I; ++K) s0 += tile_row4(tile_ptr(band, tb, I, K), ring[K % RING]);
-    const double s = row16_sum((s0 + s1) + (s2 + s3));
-    if (q == 0) st[i] = b[I * T + piv[I * T + i]] - s;
-    __syncthreads();
-    const double u = row16_sum(tile_row4(linv + I * (int64_t)T * T, st));
+    const double s = row16_sum(band_row_sum(band, tb, I, K0, I, ring));
     if (q == 0) {
+      const double u = y[I * T + i] - s;
       ring[I % RING][i] = u;
       y[I * T + i] = u;
     }
@@ -340,29 +398,17 @@ __global__ __launch_bounds__(SW) void k_fwd_sweep(int64_t nt, int tb, const S *_
   }
 }
 
+// x_I = d_I - Σ_{J>I} U~_IJ x_J  (d precomputed into x)
 template <class S>
 __global__ __launch_bounds__(SW) void k_bwd_sweep(int64_t nt, int tb, const S *__restrict__ band,
-                                                  const S *__restrict__ uinv,
                                                   double *__restrict__ x) {
   __shared__ double ring[RING][T];
-  __shared__ double st[T];
   const int i = threadIdx.x >> 4, q = threadIdx.x & 15;
   for (int64_t I = nt - 1; I >= 0; --I) {
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
     const int64_t J1 = I + tb > nt - 1 ? nt - 1 : I + tb;
-    int64_t J = I + 1;
-    for (; J + 3 <= J1; J += 4) {
-      s0 += tile_row4(tile_ptr(band, tb, I, J), ring[J % RING]);
-      s1 += tile_row4(tile_ptr(band, tb, I, J + 1), ring[(J + 1) % RING]);
-      s2 += tile_row4(tile_ptr(band, tb, I, J + 2), ring[(J + 2) % RING]);
-      s3 += tile_row4(tile_ptr(band, tb, I, J + 3), ring[(J + 3) % RING]);
-    }
-    for (; J <= J1; ++J) s0 += tile_row4(tile_ptr(band, tb, I, J), ring[J % RING]);
-    const double s = row16_sum((s0 + s1) + (s2 + s3));
-    if (q == 0) st[i] = x[I * T + i] - s;
-    __syncthreads();
-    const double u = row16_sum(tile_row4(uinv + I * (int64_t)T * T, st));
+    const double s = row16_sum(band_row_sum(band, tb, I, I + 1, J1 + 1, ring));
     if (q == 0) {
+      const double u = x[I * T + i] - s;
       ring[I % RING][i] = u;
       x[I * T + i] = u;
     }
@@ -391,20 +437,27 @@ static void fill_t(Ctx &c, S *band) {
 template <class S>
 static void factor_t(Ctx &c, S *band, S *li, S *ui) {
   for (int64_t K = 0; K < c.nt; ++K) {
-    hipLaunchKernelGGL(k_diag<S>, dim3(1), dim3(256), 0, c.stream, K, c.tb, band, li, ui, c.d_piv);
-    hipLaunchKernelGGL(k_panel<S>, dim3(3 * c.tb), dim3(256), 0, c.stream, K, c.nt, c.tb, band, li,
-                       ui, c.d_piv);
+    hipLaunchKernelGGL(k_diag<S>, dim3(1), dim3(64), 0, c.stream, K, c.tb, band, c.d_piv);
+    hipLaunchKernelGGL(k_panel<S>, dim3(3 * c.tb), dim3(64), 0, c.stream, K, c.nt, c.tb, band,
+                       c.d_piv);
     hipLaunchKernelGGL(k_update<S>, dim3(c.tb * c.tb), dim3(256), 0, c.stream, K, c.nt, c.tb, band);
   }
+  hipLaunchKernelGGL(k_tile_inv<S>, dim3((unsigned)c.nt), dim3(64), 0, c.stream, c.tb, band, li, ui);
+  hipLaunchKernelGGL(k_fold<S>, dim3((unsigned)(c.nt * (2 * c.tb + 1))), dim3(256), 0, c.stream,
+                     c.nt, c.tb, band, li, ui);
   MAFEM_HIP(hipGetLastError());
 }
 
 template <class S>
 static void solve_t(Ctx &c, const S *band, const S *li, const S *ui, const double *d_rhs,
                     double *d_x) {
-  hipLaunchKernelGGL(k_fwd_sweep<S>, dim3(1), dim3(SW), 0, c.stream, c.nt, c.tb, band, li, c.d_piv,
-                     d_rhs, d_x);
-  hipLaunchKernelGGL(k_bwd_sweep<S>, dim3(1), dim3(SW), 0, c.stream, c.nt, c.tb, band, ui, d_x);
+  // y = L^-1 P b in place in d_x, then x = U^-1 y
+  hipLaunchKernelGGL(k_diag_apply<S>, dim3((unsigned)c.nt), dim3(256), 0, c.stream, li, c.d_piv,
+                     d_rhs, c.d_t2);
+  hipLaunchKernelGGL(k_fwd_sweep<S>, dim3(1), dim3(SW), 0, c.stream, c.nt, c.tb, band, c.d_t2);
+  hipLaunchKernelGGL(k_diag_apply<S>, dim3((unsigned)c.nt), dim3(256), 0, c.stream, ui,
+                     (const int32_t *)nullptr, c.d_t2, d_x);
+  hipLaunchKernelGGL(k_bwd_sweep<S>, dim3(1), dim3(SW), 0, c.stream, c.nt, c.tb, band, d_x);
   MAFEM_HIP(hipGetLastError());
 }
 

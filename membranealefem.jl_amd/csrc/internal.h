@@ -85,6 +85,7 @@ struct Ctx {
   // extended vectors (numnp ndf)
   int64_t nx = 0;
   double *d_r = nullptr, *d_x = nullptr, *d_w = nullptr, *d_t = nullptr, *d_b = nullptr;
+  double *d_t2 = nullptr;        // band-solve intermediate
   double *d_red = nullptr;       // reduction scratch
   double *h_red = nullptr;       // pinned host mirror of reduction results
 
