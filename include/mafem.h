@@ -106,6 +106,8 @@ typedef struct {
   int32_t newton_iters, refine_sweeps;
   int32_t n_assemble, n_band_solve;  /* launches of the assembly / triangular-solve sweeps */
   int32_t n_factor, gmres_iters;     /* band factorizations / Krylov iterations */
+  double diag_kernel_us;           /* mean duration of the diagonal-tile LU kernel
+                                      (its middle launch in each factorization) */
   double last_relres;              /* |K du + r| / |r| of the last solve */
 } mafem_stats;
 

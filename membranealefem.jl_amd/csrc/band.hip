@@ -437,7 +437,13 @@ static void fill_t(Ctx &c, S *band) {
 template <class S>
 static void factor_t(Ctx &c, S *band, S *li, S *ui) {
   for (int64_t K = 0; K < c.nt; ++K) {
+    const bool timed = (K == c.nt / 2);  // one representative launch per factorization
+    if (timed) MAFEM_HIP(hipEventRecord(c.ev[10], c.stream));
     hipLaunchKernelGGL(k_diag<S>, dim3(1), dim3(64), 0, c.stream, K, c.tb, band, c.d_piv);
+    if (timed) {
+      MAFEM_HIP(hipEventRecord(c.ev[11], c.stream));
+      c.diag_timed = true;
+    }
     hipLaunchKernelGGL(k_panel<S>, dim3(3 * c.tb), dim3(64), 0, c.stream, K, c.nt, c.tb, band,
                        c.d_piv);
     hipLaunchKernelGGL(k_update<S>, dim3(c.tb * c.tb), dim3(256), 0, c.stream, K, c.nt, c.tb, band);
