@@ -62,17 +62,27 @@ __global__ void k_band_pad(int64_t nx, int64_t npad, int tb, S *__restrict__ ban
 
 // ------------------------------------------------------------ factorization
 
+// Maximum over the wavefront of a 32-bit key, delivered as a scalar: DPP
+// row-local steps, then row broadcasts that leave the total in lane 63.
+__device__ __forceinline__ unsigned wave_max_u32(unsigned x) {
+  x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0xB1, 0xF, 0xF, false));
+  x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x4E, 0xF, 0xF, false));
+  x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x141, 0xF, 0xF, false));
+  x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x140, 0xF, 0xF, false));
+  x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x142, 0xA, 0xF, false));
+  x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x143, 0xC, 0xF, false));
+  return (unsigned)__builtin_amdgcn_readlane((int)x, 63);
+}
+
+// Pivot key of a candidate row: |a| rounded to FP32 (order-preserving as an
+// unsigned integer) with the low 6 bits replaced by 63 - lane, so the largest
+// magnitude wins and near-ties (relative 2^-17) go to the lowest row index.
+// Rows already chosen carry key 0 and never win.
 template <class S>
-__device__ __forceinline__ void wave_argmax(S &v, int &idx) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const S ov = __shfl_xor(v, off, 64);
-    const int oi = __shfl_xor(idx, off, 64);
-    if (ov > v || (ov == v && oi < idx)) {
-      v = ov;
-      idx = oi;
-    }
-  }
+__device__ __forceinline__ unsigned pivot_key(S a, bool candidate, int lane) {
+  unsigned fb = __float_as_uint((float)fabs(a));
+  fb = fb < 64u ? 64u : fb;
+  return candidate ? ((fb & ~63u) | (unsigned)(63 - lane)) : 0u;
 }
 
 // value of x held by lane l (l wave-uniform), delivered as a scalar
@@ -87,86 +97,63 @@ __device__ __forceinline__ float lane_bcast(float x, int l) {
 }
 
 // LU with partial pivoting of the 64x64 diagonal tile in one wavefront: lane
-// i holds row i in registers.  Rows are not moved during the elimination: at
-// step j the largest |a_ij| among the rows not yet chosen is the pivot, its
-// row is broadcast element by element with readlane, and every other
-// unchosen row is eliminated.  The row chosen at step j is stored as row j of
-// the factored tile (P A = L U) and piv[K][j] records its original index.
+// i holds row i in registers (staged through LDS so the global accesses are
+// coalesced).  Rows are not moved during the elimination: at step j the
+// candidate row with the largest |a_ij| (pivot_key) is the pivot, its row is
+// broadcast element by element with readlane, and every other candidate row
+// is eliminated (non-candidates use a zero multiplier, which leaves them
+// unchanged).  The row chosen at step j is stored as row j of the factored
+// tile (P A = L U) and piv[K][j] records its original index.
 template <class S>
 __global__ __launch_bounds__(64) void k_diag(int64_t K, int tb, S *__restrict__ band,
                                              int32_t *__restrict__ piv) {
+  __shared__ S st[T][T + 1];
   const int lane = threadIdx.x;
   S *A = tile_ptr(band, tb, K, K);
+#pragma unroll 8
+  for (int i = 0; i < T; ++i) st[i][lane] = A[i * T + lane];
+  __syncthreads();
   S r[T];
 #pragma unroll
-  for (int k = 0; k < T; ++k) r[k] = A[lane * T + k];
+  for (int k = 0; k < T; ++k) r[k] = st[lane][k];
   int pos = -1;
 #pragma unroll
   for (int j = 0; j < T; ++j) {
-    S v = (pos < 0) ? fabs(r[j]) : S(-1);
-    int p = lane;
-    wave_argmax(v, p);
-    p = __builtin_amdgcn_readfirstlane(p);
-    if (lane == p) pos = j;
-    const bool elim = pos < 0;
+    const unsigned key = wave_max_u32(pivot_key(r[j], pos < 0, lane));
+    const int p = 63 - (int)(key & 63u);
     const S pj = lane_bcast(r[j], p);
-    const S l = r[j] / pj;
-    if (elim) r[j] = l;
+    if (lane == p) pos = j;
+    const S l = (pos < 0) ? r[j] / pj : S(0);
+    if (pos < 0) r[j] = l;
 #pragma unroll
-    for (int k = j + 1; k < T; ++k) {
-      const S pk = lane_bcast(r[k], p);
-      if (elim) r[k] = fma(-l, pk, r[k]);
-    }
+    for (int k = j + 1; k < T; ++k) r[k] = fma(-l, lane_bcast(r[k], p), r[k]);
   }
+  __syncthreads();
 #pragma unroll
-  for (int k = 0; k < T; ++k) A[pos * T + k] = r[k];
+  for (int k = 0; k < T; ++k) st[pos][k] = r[k];
   piv[K * T + pos] = lane;
+  __syncthreads();
+#pragma unroll 8
+  for (int i = 0; i < T; ++i) A[i * T + lane] = st[i][lane];
 }
 
 // Panel of step K, one wavefront per tile.  kind 0: U_KJ = L_KK^-1 P_K A_KJ
 // (lane = column, forward substitution); kind 1: L_IK = A_IK U_KK^-1 (lane =
-// row, substitution against U); kind 2: P_K applied to the L tiles already
-// stored left of the diagonal in tile row K.  The factored diagonal tile is
-// read with wave-uniform addresses.
+// row, substitution against U, the tile staged through LDS for coalescing);
+// kind 2: P_K applied to the L tiles already stored left of the diagonal in
+// tile row K.  The factored diagonal tile is staged in LDS and read with
+// wave-uniform (broadcast) addresses.
 template <class S>
 __global__ __launch_bounds__(64) void k_panel(int64_t K, int64_t nt, int tb, S *__restrict__ band,
                                               const int32_t *__restrict__ piv) {
+  __shared__ S sD[T][T];
+  __shared__ S sA[T][T + 1];
+  __shared__ S rdiag[T];
   const int lane = threadIdx.x;
   const int kind = blockIdx.x / tb, off = blockIdx.x % tb;
   const int32_t *pv = piv + K * T;
-  const S *D = tile_ptr(band, tb, K, K);
   S x[T];
-  if (kind == 0) {
-    const int64_t J = K + 1 + off;
-    if (J >= nt) return;
-    S *A = tile_ptr(band, tb, K, J);
-#pragma unroll
-    for (int i = 0; i < T; ++i) x[i] = A[(int64_t)pv[i] * T + lane];
-#pragma unroll
-    for (int i = 1; i < T; ++i) {
-      S s = x[i];
-#pragma unroll
-      for (int k = 0; k < i; ++k) s = fma(-D[i * T + k], x[k], s);
-      x[i] = s;
-    }
-#pragma unroll
-    for (int i = 0; i < T; ++i) A[i * T + lane] = x[i];
-  } else if (kind == 1) {
-    const int64_t I = K + 1 + off;
-    if (I >= nt) return;
-    S *A = tile_ptr(band, tb, I, K);
-#pragma unroll
-    for (int k = 0; k < T; ++k) x[k] = A[lane * T + k];
-#pragma unroll
-    for (int j = 0; j < T; ++j) {
-      S s = x[j];
-#pragma unroll
-      for (int k = 0; k < j; ++k) s = fma(-x[k], D[k * T + j], s);
-      x[j] = s / D[j * T + j];
-    }
-#pragma unroll
-    for (int k = 0; k < T; ++k) A[lane * T + k] = x[k];
-  } else {
+  if (kind == 2) {
     const int64_t J = K - tb + off;
     if (J < 0) return;
     S *A = tile_ptr(band, tb, K, J);
@@ -174,6 +161,46 @@ __global__ __launch_bounds__(64) void k_panel(int64_t K, int64_t nt, int tb, S *
     for (int i = 0; i < T; ++i) x[i] = A[(int64_t)pv[i] * T + lane];
 #pragma unroll
     for (int i = 0; i < T; ++i) A[i * T + lane] = x[i];
+    return;
+  }
+  const int64_t IJ = K + 1 + off;
+  if (IJ >= nt) return;
+  const S *D = tile_ptr(band, tb, K, K);
+#pragma unroll 8
+  for (int i = 0; i < T; ++i) sD[i][lane] = D[i * T + lane];
+  if (kind == 0) {
+    S *A = tile_ptr(band, tb, K, IJ);
+#pragma unroll
+    for (int i = 0; i < T; ++i) x[i] = A[(int64_t)pv[i] * T + lane];
+    __syncthreads();
+    // right-looking: once x_k is final it is eliminated from every later row,
+    // so the updates of one step are independent of each other
+#pragma unroll
+    for (int k = 0; k < T - 1; ++k)
+#pragma unroll
+      for (int i = k + 1; i < T; ++i) x[i] = fma(-sD[i][k], x[k], x[i]);
+#pragma unroll
+    for (int i = 0; i < T; ++i) A[i * T + lane] = x[i];
+  } else {
+    S *A = tile_ptr(band, tb, IJ, K);
+#pragma unroll 8
+    for (int i = 0; i < T; ++i) sA[i][lane] = A[i * T + lane];
+    __syncthreads();
+    rdiag[lane] = S(1) / sD[lane][lane];
+#pragma unroll
+    for (int k = 0; k < T; ++k) x[k] = sA[lane][k];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < T; ++j) {
+      x[j] *= rdiag[j];
+#pragma unroll
+      for (int k = j + 1; k < T; ++k) x[k] = fma(-x[j], sD[j][k], x[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < T; ++k) sA[lane][k] = x[k];
+    __syncthreads();
+#pragma unroll 8
+    for (int i = 0; i < T; ++i) A[i * T + lane] = sA[i][lane];
   }
 }
 
@@ -207,6 +234,13 @@ __global__ __launch_bounds__(256) void k_update(int64_t K, int64_t nt, int tb, S
   if (I >= nt || J >= nt) return;
   const S *Lt = tile_ptr(band, tb, I, K);
   const S *Ut = tile_ptr(band, tb, K, J);
+  S *A = tile_ptr(band, tb, I, J);
+  const int i0 = (threadIdx.x / 16) * 4, j0 = (threadIdx.x % 16) * 4;
+  S old[4][4];  // the target tile is fetched together with the operands
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) old[a][b] = A[(i0 + a) * T + j0 + b];
   for (int k = threadIdx.x; k < T * T; k += 256) {
     sa[k / T][k % T] = Lt[k];
     sb[k / T][k % T] = Ut[k];
@@ -214,12 +248,10 @@ __global__ __launch_bounds__(256) void k_update(int64_t K, int64_t nt, int tb, S
   __syncthreads();
   S c[4][4];
   tile_mm<S>(sa, sb, c);
-  S *A = tile_ptr(band, tb, I, J);
-  const int i0 = (threadIdx.x / 16) * 4, j0 = (threadIdx.x % 16) * 4;
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) A[(i0 + a) * T + j0 + b] -= c[a][b];
+    for (int b = 0; b < 4; ++b) A[(i0 + a) * T + j0 + b] = old[a][b] - c[a][b];
 }
 
 // ------------------------------------------------- post-factorization pass
