@@ -167,6 +167,60 @@ int mafem_get_stats(const mafem_ctx *ctx, mafem_stats *out);
 int mafem_set_solver(mafem_ctx *ctx, const mafem_solver_opts *opts);
 int mafem_get_solver(const mafem_ctx *ctx, mafem_solver_opts *opts);
 
+/* ------------------------------------------------------------------ batches
+ *
+ * No counterpart in the reference (it advances one mesh per `solve` call,
+ * src/MembraneAleFem.jl:44-60).  A batch advances many independent contexts
+ * (patches of any size and formulation) through one time step together:
+ * each phase is one launch covering every patch, and the Newton iteration's
+ * launch sequence is recorded once in HIP graphs and replayed.  The Newton
+ * step of each patch keeps the reference's definition (assembly, solve,
+ * update, ε = |Δu|_2 / nmdf < εnr, at most max_iter iterations;
+ * FiniteElement.jl:11-63); its linear solve is FP64 GMRES(m), right-
+ * preconditioned by a band LU stored in `factor_precision` (32: the
+ * mixed-precision path) and kept across iterations and steps; a patch is
+ * refactored when a solve needs more than `refactor_iters` Arnoldi steps or
+ * does not converge in `max_cycles` restart cycles.  The batch borrows
+ * its members: they must outlive it, and while they belong to it only their
+ * state calls (set_state, get_state, set_prescribed) may be used on them. */
+typedef struct mafem_batch mafem_batch;
+
+typedef struct {
+  int32_t factor_precision;  /* 32 (default) or 64 */
+  int32_t restart;           /* GMRES basis size m (default 20, at most 64) */
+  int32_t max_cycles;        /* restart cycles per solve before refactoring (default 3) */
+  int32_t refactor_iters;    /* refactor when a solve needed more Arnoldi steps (default 12) */
+  int32_t use_graphs;        /* 1 (default): replay recorded HIP graphs */
+  int32_t reserved;
+  double rtol;               /* target |b - K x| / |b| of each solve (default 1e-12) */
+} mafem_batch_opts;
+
+typedef struct {
+  double total_us, assemble_us, factor_us, solve_us, update_us;  /* last time step */
+  int32_t newton_iters;      /* largest per-patch Newton iteration count */
+  int32_t factor_passes;     /* batched factorization passes */
+  int32_t gmres_cycles;      /* batched GMRES restart cycles */
+  int32_t hard_failures;     /* solves left unconverged after a fresh factorization */
+  int64_t patch_iters;       /* sum over patches of Newton iterations */
+  int64_t patch_factors;     /* sum over patches of factorizations */
+  int64_t gmres_iters;       /* sum over patches of Arnoldi steps */
+  int64_t elements;          /* sum over patches of area elements */
+} mafem_batch_stats;
+
+/* opts may be NULL (defaults).  All members must live on the current device. */
+int mafem_batch_create(mafem_batch **out, int32_t n, mafem_ctx *const *members,
+                       const mafem_batch_opts *opts);
+void mafem_batch_destroy(mafem_batch *b);
+const char *mafem_batch_last_error(const mafem_batch *b);
+/* One backward-Euler step of every member: the predictor update_xms! when
+ * `predict` is nonzero (Analysis.jl:70), then Newton.  time and dt hold one
+ * value per member; iters[n], eps_hist[n * max_iter] (row per member) and
+ * status[n] (0 converged, 1 not converged) may be NULL.  Returns 1 if any
+ * member did not converge. */
+int mafem_batch_time_step(mafem_batch *b, const double *time, const double *dt, int32_t predict,
+                          int32_t max_iter, int32_t *iters, double *eps_hist, int32_t *status);
+int mafem_batch_get_stats(const mafem_batch *b, mafem_batch_stats *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -6,7 +6,7 @@ from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SOURCES = ["assemble.hip", "band.hip", "vec.hip", "krylov.hip", "abi.hip"]
+SOURCES = ["assemble.hip", "band.hip", "vec.hip", "krylov.hip", "abi.hip", "batch.hip"]
 FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC",
          # packed FP32 instructions stay disabled on this target
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

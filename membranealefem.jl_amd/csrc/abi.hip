@@ -10,9 +10,6 @@
 
 using namespace mafem;
 
-struct mafem_ctx {
-  Ctx c;
-};
 
 static thread_local std::string g_create_err;
 
@@ -271,28 +268,11 @@ void build(Ctx &c, const mafem_mesh_desc *m, const mafem_params *p) {
       }
   }
 
-  // CSC export over active DOFs, columns in equation order
-  c.h_colptr.assign(nmdf + 1, 1);
-  c.h_rowval.clear();
-  std::vector<int64_t> csc_src;
-  for (int64_t j = 0; j < nmdf; ++j) {
-    c.h_colptr[j] = (int64_t)c.h_rowval.size() + 1;
-    const int64_t kx = eq_ext[j], q = kx / ndf, cc = kx % ndf;
-    const int64_t q1 = q % n1, q2 = q / n1;
-    for (int64_t p2 = std::max<int64_t>(0, q2 - 2); p2 <= std::min(n2 - 1, q2 + 2); ++p2)
-      for (int64_t p1 = std::max<int64_t>(0, q1 - 2); p1 <= std::min(n1 - 1, q1 + 2); ++p1) {
-        const int64_t pp = p1 + n1 * p2;
-        const int64_t bk = blk_of(pp, q);
-        for (int r = 0; r < ndf; ++r) {
-          const int64_t id = m->ID[r + ndf * pp];
-          if (id == 0) continue;
-          c.h_rowval.push_back(id);
-          csc_src.push_back(bk * ndf * ndf + r * ndf + cc);
-        }
-      }
-  }
-  c.h_colptr[nmdf] = (int64_t)c.h_rowval.size() + 1;
-  c.nnz = (int64_t)c.h_rowval.size();
+  // kept for the CSC export, which is built on first use (ensure_csc)
+  c.h_ID.assign(m->ID, m->ID + ndf * numnp);
+  c.h_rowptr = rowptr;
+  c.h_eq = eq_ext;
+  c.nnz = -1;
 
   // band layout
   c.nx = numnp * ndf;
@@ -339,11 +319,47 @@ void build(Ctx &c, const mafem_mesh_desc *m, const mafem_params *p) {
     MAFEM_HIP(hipMemset(v, 0, sizeof(double) * nv));
   c.d_red = dalloc<double>(c, 512);
   MAFEM_HIP(hipHostMalloc((void **)&c.h_red, sizeof(double) * 16, hipHostMallocDefault));
-  c.d_csc_src = dupload(c, csc_src);
-  c.d_csc_val = dalloc<double>(c, c.nnz);
   c.d_eq = dupload(c, eq_ext);
   c.opts = mafem_solver_opts{MAFEM_SOLVER_DIRECT, 64, 30, 200, 20, 3, 1e-13};
   c.d_piv = dalloc<int32_t>(c, c.nt * TILE);
+}
+
+// CSC export over active DOFs, columns in equation order (the pattern of
+// K_gl, FiniteElement.jl:129-136, as a structural superset)
+void ensure_csc(Ctx &c) {
+  if (c.nnz >= 0) return;
+  const int64_t n1 = c.num1np, n2 = c.num2np, nmdf = c.nmdf;
+  const int ndf = c.ndf;
+  auto blk_of = [&](int64_t p, int64_t q) -> int64_t {
+    const int64_t i1 = p % n1, i2 = p / n1, j1 = q % n1, j2 = q / n1;
+    const int64_t lo1 = std::max<int64_t>(0, i1 - 2), hi1 = std::min(n1 - 1, i1 + 2);
+    const int64_t lo2 = std::max<int64_t>(0, i2 - 2);
+    return c.h_rowptr[p] + (j2 - lo2) * (hi1 - lo1 + 1) + (j1 - lo1);
+  };
+  c.h_colptr.assign(nmdf + 1, 1);
+  c.h_rowval.clear();
+  std::vector<int64_t> csc_src;
+  for (int64_t j = 0; j < nmdf; ++j) {
+    c.h_colptr[j] = (int64_t)c.h_rowval.size() + 1;
+    const int64_t kx = c.h_eq[j], q = kx / ndf, cc = kx % ndf;
+    const int64_t q1 = q % n1, q2 = q / n1;
+    for (int64_t p2 = std::max<int64_t>(0, q2 - 2); p2 <= std::min(n2 - 1, q2 + 2); ++p2)
+      for (int64_t p1 = std::max<int64_t>(0, q1 - 2); p1 <= std::min(n1 - 1, q1 + 2); ++p1) {
+        const int64_t pp = p1 + n1 * p2;
+        const int64_t bk = blk_of(pp, q);
+        for (int r = 0; r < ndf; ++r) {
+          const int64_t id = c.h_ID[r + ndf * pp];
+          if (id == 0) continue;
+          c.h_rowval.push_back(id);
+          csc_src.push_back(bk * ndf * ndf + r * ndf + cc);
+        }
+      }
+  }
+  c.h_colptr[nmdf] = (int64_t)c.h_rowval.size() + 1;
+  const int64_t nnz = (int64_t)c.h_rowval.size();
+  c.d_csc_src = dupload(c, csc_src);
+  c.d_csc_val = dalloc<double>(c, nnz);
+  c.nnz = nnz;
 }
 
 __global__ void k_gather_idx(int64_t n, const int64_t *__restrict__ idx,
@@ -551,6 +567,7 @@ int mafem_predict(mafem_ctx *ctx, double dt) {
 int mafem_nnz(mafem_ctx *ctx, int64_t *nnz) {
   ABI_GUARD(ctx, {
     if (!nnz) return fail(c, "null nnz", MAFEM_EINVAL);
+    ensure_csc(c);
     *nnz = c.nnz;
     return MAFEM_OK;
   });
@@ -560,6 +577,7 @@ int mafem_assemble(mafem_ctx *ctx, double time, double dt, double *r, int64_t *c
                    int64_t *rowval, double *nzval) {
   ABI_GUARD(ctx, {
     MAFEM_HIP(hipSetDevice(c.dev));
+    if (colptr || rowval || nzval) ensure_csc(c);
     MAFEM_HIP(hipEventRecord(c.ev[0], c.stream));
     launch_assemble(c, time, dt);
     MAFEM_HIP(hipEventRecord(c.ev[1], c.stream));

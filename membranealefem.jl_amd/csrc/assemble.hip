@@ -92,8 +92,8 @@ __device__ __forceinline__ void geometry(const double *X /*[3][9]*/, const doubl
 }
 
 template <int MO>
-__global__ __launch_bounds__(256) void k_area(
-    int64_t numel, int64_t numnp, const int32_t *__restrict__ ix,
+__device__ __forceinline__ void area_body(
+    int64_t bid, int64_t numel, int64_t numnp, const int32_t *__restrict__ ix,
     const int32_t *__restrict__ uel, const uint8_t *__restrict__ active,
     const double *__restrict__ xms, const double *__restrict__ cps,
     const double *__restrict__ ufns, const double *__restrict__ dbt, PhysParams ph,
@@ -104,7 +104,7 @@ __global__ __launch_bounds__(256) void k_area(
   __shared__ double sC[MAXE][NDF * 9];
   __shared__ double sB[MAXE][9 * RECP];
 
-  const int64_t f0 = (int64_t)blockIdx.x * 256;
+  const int64_t f0 = bid * 256;
   const int64_t e_lo = f0 / NE;
   int64_t e_hi = (f0 + 255) / NE;
   if (e_hi > numel - 1) e_hi = numel - 1;
@@ -378,8 +378,8 @@ __global__ __launch_bounds__(256) void k_area(
 
 // Neumann edge elements (FiniteElement.jl:338-400): 3 Gauss points, v rows only.
 template <int MO>
-__global__ __launch_bounds__(256) void k_edge(
-    int64_t n_items, int64_t numnp, const EdgeItem *__restrict__ items,
+__device__ __forceinline__ void edge_body(
+    int64_t bid, int64_t n_items, int64_t numnp, const EdgeItem *__restrict__ items,
     const int32_t *__restrict__ ix, const uint8_t *__restrict__ active,
     const double *__restrict__ xms, const double *__restrict__ cps,
     const double *__restrict__ bufns, PhysParams ph, double time, double dt,
@@ -389,7 +389,7 @@ __global__ __launch_bounds__(256) void k_edge(
   __shared__ double sX[MAXE][3 * 9];
   __shared__ double sB[MAXE][3 * RECP];
 
-  const int64_t f0 = (int64_t)blockIdx.x * 256;
+  const int64_t f0 = bid * 256;
   const int64_t k_lo = f0 / NE;
   int64_t k_hi = (f0 + 255) / NE;
   if (k_hi > n_items - 1) k_hi = n_items - 1;
@@ -489,14 +489,60 @@ __global__ __launch_bounds__(256) void k_edge(
   }
 }
 
+template <int MO>
+__global__ __launch_bounds__(256) void k_area(
+    int64_t numel, int64_t numnp, const int32_t *__restrict__ ix,
+    const int32_t *__restrict__ uel, const uint8_t *__restrict__ active,
+    const double *__restrict__ xms, const double *__restrict__ cps,
+    const double *__restrict__ ufns, const double *__restrict__ dbt, PhysParams ph,
+    double dt, double *__restrict__ kel, double *__restrict__ rel) {
+  area_body<MO>(blockIdx.x, numel, numnp, ix, uel, active, xms, cps, ufns, dbt, ph, dt, kel, rel);
+}
+
+template <int MO>
+__global__ __launch_bounds__(256) void k_edge(
+    int64_t n_items, int64_t numnp, const EdgeItem *__restrict__ items,
+    const int32_t *__restrict__ ix, const uint8_t *__restrict__ active,
+    const double *__restrict__ xms, const double *__restrict__ cps,
+    const double *__restrict__ bufns, PhysParams ph, double time, double dt,
+    double *__restrict__ kel, double *__restrict__ rel) {
+  edge_body<MO>(blockIdx.x, n_items, numnp, items, ix, active, xms, cps, bufns, ph, time, dt,
+                kel, rel);
+}
+
+// Batched forms: block -> patch through the segment map; patches that are
+// off skip the launch.
+template <int MO>
+__global__ __launch_bounds__(256) void kb_area(Seg sg, const PatchDev *__restrict__ pd,
+                                               const uint8_t *__restrict__ on,
+                                               const double *__restrict__ dts) {
+  const int p = sg.map[blockIdx.x];
+  if (!on[p]) return;
+  const PatchDev &d = pd[p];
+  area_body<MO>(blockIdx.x - sg.blk0[p], d.numel, d.numnp, d.ix, d.uel, d.active, d.xms, d.cps,
+                d.ufns, d.dbt, d.ph, dts[p], d.kel, d.rel);
+}
+
+template <int MO>
+__global__ __launch_bounds__(256) void kb_edge(Seg sg, const PatchDev *__restrict__ pd,
+                                               const uint8_t *__restrict__ on,
+                                               const double *__restrict__ times,
+                                               const double *__restrict__ dts) {
+  const int p = sg.map[blockIdx.x];
+  if (!on[p]) return;
+  const PatchDev &d = pd[p];
+  edge_body<MO>(blockIdx.x - sg.blk0[p], d.n_edges, d.numnp, d.edges, d.ix, d.active, d.xms,
+                d.cps, d.bufns, d.ph, times[p], dts[p], d.kel + d.numel * (int64_t)d.ne * d.ne,
+                d.rel + d.numel * (int64_t)d.ne);
+}
+
 // Deterministic gather of element blocks into the node-block matrix of the
 // extended system: entries of prescribed DOFs become identity rows/columns.
-__global__ __launch_bounds__(256) void k_gather_bsr(
-    int64_t nnzb, int ndf, int ne, const int32_t *__restrict__ brow,
+__device__ __forceinline__ void gather_bsr_at(
+    int64_t t, int64_t nnzb, int ndf, int ne, const int32_t *__restrict__ brow,
     const int32_t *__restrict__ colidx, const uint8_t *__restrict__ active,
     const int64_t *__restrict__ gptr, const int64_t *__restrict__ gsrc,
     const double *__restrict__ kel, double *__restrict__ bsr) {
-  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int bb = ndf * ndf;
   if (t >= nnzb * bb) return;
   const int64_t blk = t / bb;
@@ -513,11 +559,11 @@ __global__ __launch_bounds__(256) void k_gather_bsr(
   bsr[t] = s;
 }
 
-__global__ __launch_bounds__(256) void k_gather_r(
-    int64_t nx, int ndf, const uint8_t *__restrict__ active,
-    const int64_t *__restrict__ rptr, const int64_t *__restrict__ rsrc,
-    const double *__restrict__ rel, double *__restrict__ r) {
-  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+__device__ __forceinline__ void gather_r_at(int64_t t, int64_t nx, int ndf,
+                                            const uint8_t *__restrict__ active,
+                                            const int64_t *__restrict__ rptr,
+                                            const int64_t *__restrict__ rsrc,
+                                            const double *__restrict__ rel, double *__restrict__ r) {
   if (t >= nx) return;
   const int64_t p = t / ndf;
   const int d = (int)(t % ndf);
@@ -528,16 +574,64 @@ __global__ __launch_bounds__(256) void k_gather_r(
   r[t] = s;
 }
 
-// x += Δt u[:, motion] over all nodes (update_xms!, FiniteElement.jl:408-423);
-// u is column-major numnp x ndf (the predictor passes cps).
-__global__ void k_predict(int64_t numnp, int m0, int m1, int m2, const double *__restrict__ u,
-                          double dt, double *__restrict__ xms) {
-  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+// x += Δt u[:, motion] at node p (update_xms!, FiniteElement.jl:408-423); u is
+// column-major numnp x ndf (the predictor passes cps).
+__device__ __forceinline__ void predict_at(int64_t p, int64_t numnp, const int32_t *mm,
+                                           const double *__restrict__ u, double dt,
+                                           double *__restrict__ xms) {
   if (p >= numnp) return;
-  const int mm[3] = {m0, m1, m2};
 #pragma unroll
   for (int i = 0; i < 3; ++i)
     if (mm[i] > 0) xms[p + numnp * i] += dt * u[p + numnp * (mm[i] - 1)];
+}
+
+__global__ __launch_bounds__(256) void k_gather_bsr(
+    int64_t nnzb, int ndf, int ne, const int32_t *__restrict__ brow,
+    const int32_t *__restrict__ colidx, const uint8_t *__restrict__ active,
+    const int64_t *__restrict__ gptr, const int64_t *__restrict__ gsrc,
+    const double *__restrict__ kel, double *__restrict__ bsr) {
+  gather_bsr_at((int64_t)blockIdx.x * 256 + threadIdx.x, nnzb, ndf, ne, brow, colidx, active, gptr,
+                gsrc, kel, bsr);
+}
+
+__global__ __launch_bounds__(256) void k_gather_r(
+    int64_t nx, int ndf, const uint8_t *__restrict__ active,
+    const int64_t *__restrict__ rptr, const int64_t *__restrict__ rsrc,
+    const double *__restrict__ rel, double *__restrict__ r) {
+  gather_r_at((int64_t)blockIdx.x * 256 + threadIdx.x, nx, ndf, active, rptr, rsrc, rel, r);
+}
+
+__global__ void k_predict(int64_t numnp, int m0, int m1, int m2, const double *__restrict__ u,
+                          double dt, double *__restrict__ xms) {
+  const int32_t mm[3] = {m0, m1, m2};
+  predict_at((int64_t)blockIdx.x * 256 + threadIdx.x, numnp, mm, u, dt, xms);
+}
+
+__global__ __launch_bounds__(256) void kb_gather_bsr(Seg sg, const PatchDev *__restrict__ pd,
+                                                     const uint8_t *__restrict__ on) {
+  const int p = sg.map[blockIdx.x];
+  if (!on[p]) return;
+  const PatchDev &d = pd[p];
+  gather_bsr_at((blockIdx.x - sg.blk0[p]) * 256 + threadIdx.x, d.nnzb, d.ndf, d.ne, d.brow,
+                d.colidx, d.active, d.gptr, d.gsrc, d.kel, d.bsr);
+}
+
+__global__ __launch_bounds__(256) void kb_gather_r(Seg sg, const PatchDev *__restrict__ pd,
+                                                   const uint8_t *__restrict__ on) {
+  const int p = sg.map[blockIdx.x];
+  if (!on[p]) return;
+  const PatchDev &d = pd[p];
+  gather_r_at((blockIdx.x - sg.blk0[p]) * 256 + threadIdx.x, d.nx, d.ndf, d.active, d.rptr,
+              d.rsrc, d.rel, d.r);
+}
+
+__global__ __launch_bounds__(256) void kb_predict(Seg sg, const PatchDev *__restrict__ pd,
+                                                  const uint8_t *__restrict__ on,
+                                                  const double *__restrict__ dts) {
+  const int p = sg.map[blockIdx.x];
+  if (!on[p]) return;
+  const PatchDev &d = pd[p];
+  predict_at((blockIdx.x - sg.blk0[p]) * 256 + threadIdx.x, d.numnp, d.mm, d.cps, dts[p], d.xms);
 }
 
 static hipError_t launch_area(Ctx &c, double dt) {
@@ -597,6 +691,36 @@ int launch_assemble(Ctx &c, double time, double dt) {
 void launch_predict(Ctx &c, const double *d_u, double dt) {
   hipLaunchKernelGGL(k_predict, dim3((unsigned)((c.numnp + 255) / 256)), dim3(256), 0, c.stream,
                      c.numnp, c.mm_order[0], c.mm_order[1], c.mm_order[2], d_u, dt, c.d_xms);
+  MAFEM_HIP(hipGetLastError());
+}
+
+void b_assemble(BatchPlan &B) {
+  for (size_t k = 0; k < B.motions.size(); ++k) {
+    const Seg &sa = B.area[k], &se = B.edge[k];
+#define BAREA(MO)                                                                              \
+  hipLaunchKernelGGL(kb_area<MO>, dim3(sa.nblk), dim3(256), 0, B.stream, sa, B.d_pd, B.f.on,   \
+                     B.f.dt);                                                                  \
+  if (se.nblk)                                                                                 \
+    hipLaunchKernelGGL(kb_edge<MO>, dim3(se.nblk), dim3(256), 0, B.stream, se, B.d_pd, B.f.on, \
+                       B.f.time, B.f.dt)
+    switch (B.motions[k]) {
+      case LAG: BAREA(LAG); break;
+      case EUL: BAREA(EUL); break;
+      case ALEV: BAREA(ALEV); break;
+      case ALEVB: BAREA(ALEVB); break;
+      default: throw HipError(hipErrorInvalidValue, "batch motion", __FILE__, __LINE__);
+    }
+#undef BAREA
+    MAFEM_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(kb_gather_bsr, dim3(B.bsr.nblk), dim3(256), 0, B.stream, B.bsr, B.d_pd, B.f.on);
+  hipLaunchKernelGGL(kb_gather_r, dim3(B.vec.nblk), dim3(256), 0, B.stream, B.vec, B.d_pd, B.f.on);
+  MAFEM_HIP(hipGetLastError());
+}
+
+void b_predict(BatchPlan &B) {
+  hipLaunchKernelGGL(kb_predict, dim3(B.node.nblk), dim3(256), 0, B.stream, B.node, B.d_pd, B.f.on,
+                     B.f.dt);
   MAFEM_HIP(hipGetLastError());
 }
 

@@ -38,18 +38,27 @@ __global__ __launch_bounds__(256) void k_zero_t(int64_t n, S *__restrict__ x) {
 }
 
 template <class S>
-__global__ __launch_bounds__(256) void k_bsr_to_band(int64_t nnzb, int ndf, int tb,
+__device__ __forceinline__ void bsr_to_band_body(int64_t bid, int64_t nnzb, int ndf, int tb,
                                                      const int32_t *__restrict__ brow,
                                                      const int32_t *__restrict__ colidx,
                                                      const double *__restrict__ bsr,
                                                      S *__restrict__ band) {
-  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t t = (int64_t)bid * 256 + threadIdx.x;
   const int bb = ndf * ndf;
   if (t >= nnzb * bb) return;
   const int64_t blk = t / bb;
   const int rc = (int)(t % bb), r = rc / ndf, c = rc % ndf;
   const int64_t R = (int64_t)brow[blk] * ndf + r, C = (int64_t)colidx[blk] * ndf + c;
   tile_ptr(band, tb, R / T, C / T)[(R % T) * T + (C % T)] = (S)bsr[t];
+}
+
+template <class S>
+__global__ __launch_bounds__(256) void k_bsr_to_band(int64_t nnzb, int ndf, int tb,
+                                                     const int32_t *__restrict__ brow,
+                                                     const int32_t *__restrict__ colidx,
+                                                     const double *__restrict__ bsr,
+                                                     S *__restrict__ band) {
+  bsr_to_band_body<S>((int64_t)blockIdx.x, nnzb, ndf, tb, brow, colidx, bsr, band);
 }
 
 // identity on the padding rows beyond the extended system
@@ -105,7 +114,7 @@ __device__ __forceinline__ float lane_bcast(float x, int l) {
 // unchanged).  The row chosen at step j is stored as row j of the factored
 // tile (P A = L U) and piv[K][j] records its original index.
 template <class S>
-__global__ __launch_bounds__(64) void k_diag(int64_t K, int tb, S *__restrict__ band,
+__device__ __forceinline__ void diag_body(int64_t K, int tb, S *__restrict__ band,
                                              int32_t *__restrict__ piv) {
   __shared__ S st[T][T + 1];
   const int lane = threadIdx.x;
@@ -137,6 +146,12 @@ __global__ __launch_bounds__(64) void k_diag(int64_t K, int tb, S *__restrict__ 
   for (int i = 0; i < T; ++i) A[i * T + lane] = st[i][lane];
 }
 
+template <class S>
+__global__ __launch_bounds__(64) void k_diag(int64_t K, int tb, S *__restrict__ band,
+                                             int32_t *__restrict__ piv) {
+  diag_body<S>(K, tb, band, piv);
+}
+
 // Panel of step K, one wavefront per tile.  kind 0: U_KJ = L_KK^-1 P_K A_KJ
 // (lane = column, forward substitution); kind 1: L_IK = A_IK U_KK^-1 (lane =
 // row, substitution against U, the tile staged through LDS for coalescing);
@@ -144,13 +159,13 @@ __global__ __launch_bounds__(64) void k_diag(int64_t K, int tb, S *__restrict__ 
 // tile row K.  The factored diagonal tile is staged in LDS and read with
 // wave-uniform (broadcast) addresses.
 template <class S>
-__global__ __launch_bounds__(64) void k_panel(int64_t K, int64_t nt, int tb, S *__restrict__ band,
+__device__ __forceinline__ void panel_body(int64_t bid, int64_t K, int64_t nt, int tb, S *__restrict__ band,
                                               const int32_t *__restrict__ piv) {
   __shared__ S sD[T][T];
   __shared__ S sA[T][T + 1];
   __shared__ S rdiag[T];
   const int lane = threadIdx.x;
-  const int kind = blockIdx.x / tb, off = blockIdx.x % tb;
+  const int kind = bid / tb, off = bid % tb;
   const int32_t *pv = piv + K * T;
   S x[T];
   if (kind == 2) {
@@ -204,6 +219,12 @@ __global__ __launch_bounds__(64) void k_panel(int64_t K, int64_t nt, int tb, S *
   }
 }
 
+template <class S>
+__global__ __launch_bounds__(64) void k_panel(int64_t K, int64_t nt, int tb, S *__restrict__ band,
+                                              const int32_t *__restrict__ piv) {
+  panel_body<S>((int64_t)blockIdx.x, K, nt, tb, band, piv);
+}
+
 // C = A B for 64x64 tiles staged in LDS; each of 256 threads owns a 4x4 block.
 template <class S>
 __device__ __forceinline__ void tile_mm(const S (*sa)[T], const S (*sb)[T], S c[4][4]) {
@@ -227,10 +248,10 @@ __device__ __forceinline__ void tile_mm(const S (*sa)[T], const S (*sb)[T], S c[
 }
 
 template <class S>
-__global__ __launch_bounds__(256) void k_update(int64_t K, int64_t nt, int tb, S *__restrict__ band) {
+__device__ __forceinline__ void update_body(int64_t bid, int64_t K, int64_t nt, int tb, S *__restrict__ band) {
   __shared__ S sa[T][T];
   __shared__ S sb[T][T];
-  const int64_t I = K + 1 + blockIdx.x / tb, J = K + 1 + blockIdx.x % tb;
+  const int64_t I = K + 1 + bid / tb, J = K + 1 + bid % tb;
   if (I >= nt || J >= nt) return;
   const S *Lt = tile_ptr(band, tb, I, K);
   const S *Ut = tile_ptr(band, tb, K, J);
@@ -254,6 +275,11 @@ __global__ __launch_bounds__(256) void k_update(int64_t K, int64_t nt, int tb, S
     for (int b = 0; b < 4; ++b) A[(i0 + a) * T + j0 + b] = old[a][b] - c[a][b];
 }
 
+template <class S>
+__global__ __launch_bounds__(256) void k_update(int64_t K, int64_t nt, int tb, S *__restrict__ band) {
+  update_body<S>((int64_t)blockIdx.x, K, nt, tb, band);
+}
+
 // ------------------------------------------------- post-factorization pass
 //
 // Inverts every factored diagonal tile (one wavefront per tile, lane = column
@@ -263,9 +289,9 @@ __global__ __launch_bounds__(256) void k_update(int64_t K, int64_t nt, int tb, S
 // x_I = U_II^-1 y_I - Σ U~_IJ x_J.  Every tile is independent: fully parallel.
 
 template <class S>
-__global__ __launch_bounds__(64) void k_tile_inv(int tb, const S *__restrict__ band,
+__device__ __forceinline__ void tile_inv_body(int64_t bid, int tb, const S *__restrict__ band,
                                                  S *__restrict__ linv, S *__restrict__ uinv) {
-  const int64_t I = blockIdx.x;
+  const int64_t I = bid;
   const int c = threadIdx.x;
   const S *D = tile_ptr(band, tb, I, I);
   S x[T];
@@ -293,16 +319,22 @@ __global__ __launch_bounds__(64) void k_tile_inv(int tb, const S *__restrict__ b
   for (int i = 0; i < T; ++i) Ui[i * T + c] = x[i];
 }
 
+template <class S>
+__global__ __launch_bounds__(64) void k_tile_inv(int tb, const S *__restrict__ band,
+                                                 S *__restrict__ linv, S *__restrict__ uinv) {
+  tile_inv_body<S>((int64_t)blockIdx.x, tb, band, linv, uinv);
+}
+
 // blockIdx.x = I (2 tb + 1 slots): slot < tb -> L tile (I, I - tb + slot),
 // slot > tb -> U tile (I, I - tb + slot); the diagonal slot is skipped.
 template <class S>
-__global__ __launch_bounds__(256) void k_fold(int64_t nt, int tb, S *__restrict__ band,
+__device__ __forceinline__ void fold_body(int64_t bid, int64_t nt, int tb, S *__restrict__ band,
                                               const S *__restrict__ linv,
                                               const S *__restrict__ uinv) {
   __shared__ S sa[T][T];
   __shared__ S sb[T][T];
-  const int64_t I = blockIdx.x / (2 * tb + 1);
-  const int slot = blockIdx.x % (2 * tb + 1);
+  const int64_t I = bid / (2 * tb + 1);
+  const int slot = bid % (2 * tb + 1);
   const int64_t J = I - tb + slot;
   if (slot == tb || J < 0 || J >= nt) return;
   const S *Inv = (slot < tb ? linv : uinv) + I * (int64_t)T * T;
@@ -319,6 +351,13 @@ __global__ __launch_bounds__(256) void k_fold(int64_t nt, int tb, S *__restrict_
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) A[(i0 + a) * T + j0 + b] = c[a][b];
+}
+
+template <class S>
+__global__ __launch_bounds__(256) void k_fold(int64_t nt, int tb, S *__restrict__ band,
+                                              const S *__restrict__ linv,
+                                              const S *__restrict__ uinv) {
+  fold_body<S>((int64_t)blockIdx.x, nt, tb, band, linv, uinv);
 }
 
 // ------------------------------------------------------------ triangular solves
@@ -394,12 +433,12 @@ __device__ __forceinline__ double band_row_sum(const S *band, int tb, int64_t I,
 
 // out_I = M_I v_I for every tile row (M = L^-1 with v = P b, or U^-1 with v = y)
 template <class S>
-__global__ __launch_bounds__(256) void k_diag_apply(const S *__restrict__ inv,
+__device__ __forceinline__ void diag_apply_body(int64_t bid, const S *__restrict__ inv,
                                                     const int32_t *__restrict__ piv,
                                                     const double *__restrict__ v,
                                                     double *__restrict__ out) {
   __shared__ double sv[T];
-  const int64_t I = blockIdx.x;
+  const int64_t I = bid;
   if (threadIdx.x < T) sv[threadIdx.x] = v[I * T + (piv ? piv[I * T + threadIdx.x] : threadIdx.x)];
   __syncthreads();
   const int i = threadIdx.x / 4, q = threadIdx.x % 4;
@@ -412,10 +451,18 @@ __global__ __launch_bounds__(256) void k_diag_apply(const S *__restrict__ inv,
   if (q == 0) out[I * T + i] = s;
 }
 
+template <class S>
+__global__ __launch_bounds__(256) void k_diag_apply(const S *__restrict__ inv,
+                                                    const int32_t *__restrict__ piv,
+                                                    const double *__restrict__ v,
+                                                    double *__restrict__ out) {
+  diag_apply_body<S>((int64_t)blockIdx.x, inv, piv, v, out);
+}
+
 // y_I = c_I - Σ_{K<I} L~_IK y_K  (c precomputed into y)
 template <class S>
-__global__ __launch_bounds__(SW) void k_fwd_sweep(int64_t nt, int tb, const S *__restrict__ band,
-                                                  double *__restrict__ y) {
+__device__ __forceinline__ void fwd_sweep_body(int64_t nt, int tb, const S *__restrict__ band,
+                                               double *__restrict__ y) {
   __shared__ double ring[RING][T];
   const int i = threadIdx.x >> 4, q = threadIdx.x & 15;
   for (int64_t I = 0; I < nt; ++I) {
@@ -432,8 +479,8 @@ __global__ __launch_bounds__(SW) void k_fwd_sweep(int64_t nt, int tb, const S *_
 
 // x_I = d_I - Σ_{J>I} U~_IJ x_J  (d precomputed into x)
 template <class S>
-__global__ __launch_bounds__(SW) void k_bwd_sweep(int64_t nt, int tb, const S *__restrict__ band,
-                                                  double *__restrict__ x) {
+__device__ __forceinline__ void bwd_sweep_body(int64_t nt, int tb, const S *__restrict__ band,
+                                               double *__restrict__ x) {
   __shared__ double ring[RING][T];
   const int i = threadIdx.x >> 4, q = threadIdx.x & 15;
   for (int64_t I = nt - 1; I >= 0; --I) {
@@ -446,6 +493,168 @@ __global__ __launch_bounds__(SW) void k_bwd_sweep(int64_t nt, int tb, const S *_
     }
     __syncthreads();
   }
+}
+
+template <class S>
+__global__ __launch_bounds__(SW) void k_fwd_sweep(int64_t nt, int tb, const S *__restrict__ band,
+                                                  double *__restrict__ y) {
+  fwd_sweep_body<S>(nt, tb, band, y);
+}
+
+template <class S>
+__global__ __launch_bounds__(SW) void k_bwd_sweep(int64_t nt, int tb, const S *__restrict__ band,
+                                                  double *__restrict__ x) {
+  bwd_sweep_body<S>(nt, tb, band, x);
+}
+
+// ------------------------------------------------------------ batched forms
+//
+// Patch p takes part in a factorization launch when on[p] && fac[p], and in
+// a solve launch when its flag in the given mask is set.  Factorization steps locate their patch
+// through the StepMap kernel argument; the other launches through segment
+// maps.
+
+__device__ __forceinline__ bool fac_on(const uint8_t *on, const uint8_t *fac, int p) {
+  return on[p] && fac[p];
+}
+
+// 64 KiB chunks of a patch's band storage, 16 bytes per store
+__global__ __launch_bounds__(256) void kb_zero_band(Seg sg, const PatchDev *__restrict__ pd,
+                                                    const uint8_t *__restrict__ on,
+                                                    const uint8_t *__restrict__ fac, int es) {
+  const int p = sg.map[blockIdx.x];
+  if (!fac_on(on, fac, p)) return;
+  const PatchDev &d = pd[p];
+  const int64_t bytes = d.nt * (2 * d.tb + 1) * (int64_t)T * T * es;
+  const int64_t base = (int64_t)(blockIdx.x - sg.blk0[p]) * 65536;
+  uint4 *dst = reinterpret_cast<uint4 *>(d.band);
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int64_t off = base + k * 4096 + threadIdx.x * 16;
+    if (off < bytes) dst[off / 16] = make_uint4(0u, 0u, 0u, 0u);
+  }
+}
+
+template <class S>
+__global__ __launch_bounds__(256) void kb_bsr_to_band(Seg sg, const PatchDev *__restrict__ pd,
+                                                      const uint8_t *__restrict__ on,
+                                                      const uint8_t *__restrict__ fac) {
+  const int p = sg.map[blockIdx.x];
+  if (!fac_on(on, fac, p)) return;
+  const PatchDev &d = pd[p];
+  bsr_to_band_body<S>(blockIdx.x - sg.blk0[p], d.nnzb, d.ndf, d.tb, d.brow, d.colidx, d.bsr,
+                      (S *)d.band);
+}
+
+// one block per patch: identity on the < 64 padding rows
+template <class S>
+__global__ __launch_bounds__(64) void kb_band_pad(const PatchDev *__restrict__ pd,
+                                                  const uint8_t *__restrict__ on,
+                                                  const uint8_t *__restrict__ fac) {
+  const int p = blockIdx.x;
+  if (!fac_on(on, fac, p)) return;
+  const PatchDev &d = pd[p];
+  const int64_t R = d.nx + threadIdx.x;
+  if (R < d.nt * T) tile_ptr((S *)d.band, d.tb, R / T, R / T)[(R % T) * T + (R % T)] = S(1);
+}
+
+// patch (through `order`) and block within the patch of a factorization step
+// launch; per = blocks per patch as a function of its tb
+template <int KIND>
+__device__ __forceinline__ int step_locate(const StepMap &sm, const int32_t *__restrict__ order,
+                                           int &local, int &tb) {
+  const int b = blockIdx.x;
+  int c = 0;
+  while (c + 1 < sm.ncls && b >= sm.blk0[c + 1]) ++c;
+  tb = sm.tb[c];
+  const int per = KIND == 0 ? 1 : (KIND == 1 ? 3 * tb : tb * tb);
+  const int rel = b - sm.blk0[c];
+  local = rel % per;
+  return order[sm.first[c] + rel / per];
+}
+
+template <class S>
+__global__ __launch_bounds__(64) void kb_diag(StepMap sm, int64_t K, const int32_t *__restrict__ order,
+                                              const PatchDev *__restrict__ pd,
+                                              const uint8_t *__restrict__ on,
+                                              const uint8_t *__restrict__ fac) {
+  int local, tb;
+  const int p = step_locate<0>(sm, order, local, tb);
+  if (!fac_on(on, fac, p)) return;
+  diag_body<S>(K, tb, (S *)pd[p].band, pd[p].piv);
+}
+
+template <class S>
+__global__ __launch_bounds__(64) void kb_panel(StepMap sm, int64_t K, const int32_t *__restrict__ order,
+                                               const PatchDev *__restrict__ pd,
+                                               const uint8_t *__restrict__ on,
+                                               const uint8_t *__restrict__ fac) {
+  int local, tb;
+  const int p = step_locate<1>(sm, order, local, tb);
+  if (!fac_on(on, fac, p)) return;
+  panel_body<S>(local, K, pd[p].nt, tb, (S *)pd[p].band, pd[p].piv);
+}
+
+template <class S>
+__global__ __launch_bounds__(256) void kb_update(StepMap sm, int64_t K, const int32_t *__restrict__ order,
+                                                 const PatchDev *__restrict__ pd,
+                                                 const uint8_t *__restrict__ on,
+                                                 const uint8_t *__restrict__ fac) {
+  int local, tb;
+  const int p = step_locate<2>(sm, order, local, tb);
+  if (!fac_on(on, fac, p)) return;
+  update_body<S>(local, K, pd[p].nt, tb, (S *)pd[p].band);
+}
+
+template <class S>
+__global__ __launch_bounds__(64) void kb_tile_inv(Seg sg, const PatchDev *__restrict__ pd,
+                                                  const uint8_t *__restrict__ on,
+                                                  const uint8_t *__restrict__ fac) {
+  const int p = sg.map[blockIdx.x];
+  if (!fac_on(on, fac, p)) return;
+  const PatchDev &d = pd[p];
+  tile_inv_body<S>(blockIdx.x - sg.blk0[p], d.tb, (const S *)d.band, (S *)d.linv, (S *)d.uinv);
+}
+
+template <class S>
+__global__ __launch_bounds__(256) void kb_fold(Seg sg, const PatchDev *__restrict__ pd,
+                                               const uint8_t *__restrict__ on,
+                                               const uint8_t *__restrict__ fac) {
+  const int p = sg.map[blockIdx.x];
+  if (!fac_on(on, fac, p)) return;
+  const PatchDev &d = pd[p];
+  fold_body<S>(blockIdx.x - sg.blk0[p], d.nt, d.tb, (S *)d.band, (const S *)d.linv,
+               (const S *)d.uinv);
+}
+
+// which = 0: t2 = L^-1 P w; which = 1: t = U^-1 t2 (per tile row)
+template <class S>
+__global__ __launch_bounds__(256) void kb_diag_apply(Seg sg, const PatchDev *__restrict__ pd,
+                                                     const uint8_t *__restrict__ mask, int which) {
+  const int p = sg.map[blockIdx.x];
+  if (!mask[p]) return;
+  const PatchDev &d = pd[p];
+  if (which == 0)
+    diag_apply_body<S>(blockIdx.x - sg.blk0[p], (const S *)d.linv, d.piv, d.w, d.t2);
+  else
+    diag_apply_body<S>(blockIdx.x - sg.blk0[p], (const S *)d.uinv, (const int32_t *)nullptr, d.t2,
+                       d.t);
+}
+
+template <class S>
+__global__ __launch_bounds__(SW) void kb_fwd_sweep(const PatchDev *__restrict__ pd,
+                                                   const uint8_t *__restrict__ mask) {
+  const int p = blockIdx.x;
+  if (!mask[p]) return;
+  fwd_sweep_body<S>(pd[p].nt, pd[p].tb, (const S *)pd[p].band, pd[p].t2);
+}
+
+template <class S>
+__global__ __launch_bounds__(SW) void kb_bwd_sweep(const PatchDev *__restrict__ pd,
+                                                   const uint8_t *__restrict__ mask) {
+  const int p = blockIdx.x;
+  if (!mask[p]) return;
+  bwd_sweep_body<S>(pd[p].nt, pd[p].tb, (const S *)pd[p].band, pd[p].t);
 }
 
 // ------------------------------------------------------------------ host side
@@ -525,6 +734,8 @@ static void ensure_band(Ctx &c, int prec) {
   c.band_bytes = (int64_t)(es * ntile);
 }
 
+void band_alloc(Ctx &c, int prec) { ensure_band(c, prec); }
+
 void band_fill(Ctx &c, int prec) {
   ensure_band(c, prec);
   if (prec == 32)
@@ -549,6 +760,73 @@ void band_solve(Ctx &c, const double *d_rhs, double *d_x) {
   else
     solve_t<double>(c, (const double *)c.d_band, (const double *)c.d_linv,
                     (const double *)c.d_uinv, d_rhs, d_x);
+}
+
+template <class S>
+static void b_factor_t(BatchPlan &B) {
+  const BatchFlags &f = B.f;
+  hipLaunchKernelGGL(kb_zero_band, dim3(B.zero.nblk), dim3(256), 0, B.stream, B.zero, B.d_pd, f.on,
+                     f.fac, (int)sizeof(S));
+  hipLaunchKernelGGL(kb_bsr_to_band<S>, dim3(B.bsr.nblk), dim3(256), 0, B.stream, B.bsr, B.d_pd,
+                     f.on, f.fac);
+  hipLaunchKernelGGL(kb_band_pad<S>, dim3(B.P), dim3(64), 0, B.stream, B.d_pd, f.on, f.fac);
+  MAFEM_HIP(hipGetLastError());
+  for (int64_t K = 0; K < (int64_t)B.steps.size(); ++K)
+    for (const StepMap &sm : B.steps[K]) {
+      const int32_t nd = sm.blk0[sm.ncls];
+      int32_t np = 0, nu = 0;
+      for (int c = 0; c < sm.ncls; ++c) {
+        np += sm.count[c] * 3 * sm.tb[c];
+        nu += sm.count[c] * sm.tb[c] * sm.tb[c];
+      }
+      StepMap smp = sm, smu = sm;
+      int32_t bp = 0, bu = 0;
+      for (int c = 0; c < sm.ncls; ++c) {
+        smp.blk0[c] = bp;
+        smu.blk0[c] = bu;
+        bp += sm.count[c] * 3 * sm.tb[c];
+        bu += sm.count[c] * sm.tb[c] * sm.tb[c];
+      }
+      smp.blk0[sm.ncls] = bp;
+      smu.blk0[sm.ncls] = bu;
+      hipLaunchKernelGGL(kb_diag<S>, dim3(nd), dim3(64), 0, B.stream, sm, K, B.d_order, B.d_pd,
+                         f.on, f.fac);
+      hipLaunchKernelGGL(kb_panel<S>, dim3(np), dim3(64), 0, B.stream, smp, K, B.d_order, B.d_pd,
+                         f.on, f.fac);
+      if (nu > 0)
+        hipLaunchKernelGGL(kb_update<S>, dim3(nu), dim3(256), 0, B.stream, smu, K, B.d_order,
+                           B.d_pd, f.on, f.fac);
+    }
+  hipLaunchKernelGGL(kb_tile_inv<S>, dim3(B.tiles.nblk), dim3(64), 0, B.stream, B.tiles, B.d_pd,
+                     f.on, f.fac);
+  hipLaunchKernelGGL(kb_fold<S>, dim3(B.fold.nblk), dim3(256), 0, B.stream, B.fold, B.d_pd, f.on,
+                     f.fac);
+  MAFEM_HIP(hipGetLastError());
+}
+
+void b_factor(BatchPlan &B) {
+  if (B.prec == 32)
+    b_factor_t<float>(B);
+  else
+    b_factor_t<double>(B);
+}
+
+template <class S>
+static void b_band_solve_t(BatchPlan &B, const uint8_t *mask) {
+  hipLaunchKernelGGL(kb_diag_apply<S>, dim3(B.tiles.nblk), dim3(256), 0, B.stream, B.tiles, B.d_pd,
+                     mask, 0);
+  hipLaunchKernelGGL(kb_fwd_sweep<S>, dim3(B.P), dim3(SW), 0, B.stream, B.d_pd, mask);
+  hipLaunchKernelGGL(kb_diag_apply<S>, dim3(B.tiles.nblk), dim3(256), 0, B.stream, B.tiles, B.d_pd,
+                     mask, 1);
+  hipLaunchKernelGGL(kb_bwd_sweep<S>, dim3(B.P), dim3(SW), 0, B.stream, B.d_pd, mask);
+  MAFEM_HIP(hipGetLastError());
+}
+
+void b_band_solve(BatchPlan &B, const uint8_t *mask) {
+  if (B.prec == 32)
+    b_band_solve_t<float>(B, mask);
+  else
+    b_band_solve_t<double>(B, mask);
 }
 
 }  // namespace mafem

@@ -89,8 +89,10 @@ struct Ctx {
   double *d_red = nullptr;       // reduction scratch
   double *h_red = nullptr;       // pinned host mirror of reduction results
 
-  // CSC export (1-based) over active DOFs
-  int64_t nnz = 0;
+  // CSC export (1-based) over active DOFs, built on first use (nnz < 0 before)
+  std::vector<int64_t> h_ID, h_eq;
+  std::vector<int32_t> h_rowptr;
+  int64_t nnz = -1;
   std::vector<int64_t> h_colptr, h_rowval;
   int64_t *d_csc_src = nullptr;  // BSR value index of each CSC entry
   double *d_csc_val = nullptr;
@@ -121,6 +123,114 @@ struct Ctx {
   int32_t n_band_solve = 0;     // running count of triangular-solve sweeps
 };
 
+// ---------------------------------------------------------------- batches
+//
+// A batch advances many independent patches (each a Ctx with its own device
+// tables, state and band storage) through the same launches: every batched
+// kernel maps its block to a patch through a segment map (block -> patch,
+// first block of each patch) and reads the patch's sizes and buffers from a
+// PatchDev descriptor.  Per-patch flags in device memory switch patches on
+// and off, so one recorded launch sequence serves every Newton iteration.
+
+struct PatchDev {
+  int64_t numel, numnp, nx, nnzb, n_edges, nt;
+  int32_t ndf, ne, tb, motion;
+  int32_t mm[3];
+  int32_t pad0;
+  PhysParams ph;
+  double enr, nmdf;
+  const int32_t *ix, *uel;
+  const uint8_t *active;
+  const double *ufns, *dbt, *bufns;
+  const EdgeItem *edges;
+  double *xms, *cps, *kel, *rel;
+  const int32_t *rowptr, *colidx, *brow;
+  double *bsr;
+  const int64_t *gptr, *gsrc, *rptr, *rsrc;
+  double *r, *b, *x, *w, *t, *t2;
+  void *band, *linv, *uinv;
+  int32_t *piv;
+  double *V;         // GMRES basis, m + 1 vectors of length nt TILE (batch-owned)
+};
+
+// Block -> patch map of one batched launch.
+struct Seg {
+  int32_t *map = nullptr;   // [nblk] patch of each block
+  int64_t *blk0 = nullptr;  // [P] first block of each patch (unused patches: -1)
+  unsigned nblk = 0;
+};
+
+// Patches active at one step K of the batched factorization, grouped in
+// classes of equal tile half-bandwidth; patches are listed in `order`
+// (class by class, each class by decreasing tile-row count), so the active
+// members of a class are a prefix of it.
+constexpr int MAXC = 40;
+struct StepMap {
+  int32_t ncls;
+  int32_t first[MAXC];   // index into order of the class's first patch
+  int32_t count[MAXC];   // active patches of the class at this step
+  int32_t tb[MAXC];
+  int32_t blk0[MAXC + 1];  // first block of the class in this launch
+};
+
+// Per-patch flags and scalars of a batch, all in device memory.  The linear
+// solve of each Newton iteration is right-preconditioned GMRES(m) in FP64
+// with the kept band LU as preconditioner; H, cs, sn, g, hc hold each
+// patch's small Arnoldi/Givens state.
+struct BatchFlags {
+  uint8_t *on;       // still iterating Newton
+  uint8_t *sol;      // linear solve not yet converged in this iteration
+  uint8_t *fac;      // needs a factorization before its next solve
+  uint8_t *kon;      // Arnoldi process running in the current cycle
+  uint8_t *fresh;    // factorized since its last converged solve
+  int32_t *iters;    // Newton iterations done in this time step
+  int32_t *kits;     // Arnoldi steps of this iteration's solve (all cycles)
+  int32_t *kdim;     // basis size of the current cycle
+  int32_t *nfac;     // factorizations (running count)
+  int32_t *kits_tot; // Arnoldi steps (running count)
+  double *time, *dt; // per patch
+  double *bn, *relres, *eps, *scale;
+  double *H;         // [P][m][m+1]: column j of the Hessenberg matrix at H + (p m + j)(m+1)
+  double *cs, *sn;   // [P][m] Givens rotations
+  double *g;         // [P][m+1] rotated right-hand side
+  double *hc;        // [P][m+1] projection coefficients, then the least-squares solution
+  double *eps_hist;  // [P][max_iter]
+  double *part;      // [vec blocks] per-block partial sums
+  double *part2;     // [m+1][vec blocks] per-block partial dot products
+  int32_t *count;    // [4] patches still on, solves unconverged, factorizations pending, hard failures
+};
+
+struct BatchPlan {
+  int P = 0;
+  int prec = 32;
+  int max_iter = 14;
+  int m = 20;                // GMRES basis size
+  int refactor_iters = 12;
+  double rtol = 1e-12;
+  hipStream_t stream = nullptr;
+  PatchDev *d_pd = nullptr;
+  int32_t *d_order = nullptr;
+  BatchFlags f{};
+  std::vector<int> motions;         // motions present
+  std::vector<Seg> area, edge;      // per motion present
+  Seg bsr, vec, node, zero, tiles, fold;
+  std::vector<std::vector<StepMap>> steps;  // launches of each factorization step
+  int64_t max_nt = 0;
+};
+
+// batched launch sequences (assemble.hip, band.hip, vec.hip)
+void b_assemble(BatchPlan &B);
+void b_predict(BatchPlan &B);
+void b_factor(BatchPlan &B);
+void b_band_solve(BatchPlan &B, const uint8_t *mask);  // t = LU^-1 w where mask
+void b_solve_begin(BatchPlan &B);  // sol = on; b = -r, x = 0, |b|, kits = 0
+void b_gmres_cycle(BatchPlan &B);  // one restart cycle for patches with sol
+void b_solve_end(BatchPlan &B);    // unconverged solves: refactor and redo, or give up
+void b_update(BatchPlan &B);       // refactor policy, state update, ε, Newton flags, counts
+void b_mark_factored(BatchPlan &B);
+void b_count(BatchPlan &B);        // count[0] = on, count[2] = on && fac
+void band_alloc(Ctx &c, int prec); // band storage of the given precision
+
 // kernels / launchers (assemble.hip)
 int launch_assemble(Ctx &c, double time, double dt);
 void launch_predict(Ctx &c, const double *d_u, double dt);
@@ -143,6 +253,10 @@ void ensure_krylov(Ctx &c);
 void apply_update(Ctx &c, const double *d_du, double dt);
 
 }  // namespace mafem
+
+struct mafem_ctx {
+  mafem::Ctx c;
+};
 
 #define MAFEM_HIP(call)                                                         \
   do {                                                                          \

@@ -68,6 +68,27 @@ class SolverOpts(ctypes.Structure):
     ]
 
 
+class BatchOpts(ctypes.Structure):
+    _fields_ = [
+        ("factor_precision", ctypes.c_int32), ("restart", ctypes.c_int32),
+        ("max_cycles", ctypes.c_int32), ("refactor_iters", ctypes.c_int32),
+        ("use_graphs", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("rtol", ctypes.c_double),
+    ]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [
+        ("total_us", ctypes.c_double), ("assemble_us", ctypes.c_double),
+        ("factor_us", ctypes.c_double), ("solve_us", ctypes.c_double),
+        ("update_us", ctypes.c_double),
+        ("newton_iters", ctypes.c_int32), ("factor_passes", ctypes.c_int32),
+        ("gmres_cycles", ctypes.c_int32), ("hard_failures", ctypes.c_int32),
+        ("patch_iters", ctypes.c_int64), ("patch_factors", ctypes.c_int64),
+        ("gmres_iters", ctypes.c_int64), ("elements", ctypes.c_int64),
+    ]
+
+
 # every symbol include/mafem.h declares, with its ctypes signature
 SIGNATURES = {
     "mafem_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(MeshDesc),
@@ -87,6 +108,14 @@ SIGNATURES = {
     "mafem_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Stats)]),
     "mafem_set_solver": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SolverOpts)]),
     "mafem_get_solver": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SolverOpts)]),
+    "mafem_batch_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int32,
+                                          ctypes.POINTER(ctypes.c_void_p),
+                                          ctypes.POINTER(BatchOpts)]),
+    "mafem_batch_destroy": (None, [ctypes.c_void_p]),
+    "mafem_batch_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "mafem_batch_time_step": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p, ctypes.c_int32,
+                                             ctypes.c_int32, i32p, f64p, i32p]),
+    "mafem_batch_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(BatchStats)]),
 }
 
 _lib = None

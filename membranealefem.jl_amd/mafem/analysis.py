@@ -171,6 +171,64 @@ class DeviceMesh:
         return {k: getattr(s, k) for k, _ in L.Stats._fields_}
 
 
+class Batch:
+    """Many DeviceMesh contexts advanced through one time step together.
+
+    No counterpart in the reference (one mesh per `solve`); each member's
+    Newton step keeps the reference's definition (FiniteElement.jl:11-63).
+    The linear solves run as FP64 GMRES(restart) right-preconditioned by a
+    band LU kept in `precision` (32: mixed precision).  The members are
+    borrowed: keep them alive while the batch exists.
+    """
+
+    def __init__(self, members, precision=32, restart=20, max_cycles=3, refactor_iters=12,
+                 use_graphs=True, rtol=1e-12):
+        self.lib = L.load()
+        self.members = list(members)
+        n = len(self.members)
+        arr = (ctypes.c_void_p * n)(*[m.h.value for m in self.members])
+        o = L.BatchOpts(int(precision), int(restart), int(max_cycles), int(refactor_iters),
+                        1 if use_graphs else 0, 0, float(rtol))
+        h = ctypes.c_void_p()
+        rc = self.lib.mafem_batch_create(ctypes.byref(h), n, arr, ctypes.byref(o))
+        if rc != L.OK:
+            raise L.LibError(rc, self.lib.mafem_batch_last_error(None).decode())
+        self.h = h
+        self.n = n
+
+    def time_step(self, times, dts, predict=True, max_iter=14):
+        """One backward-Euler step of every member; returns (iters, eps
+        histories, status) with status 0 = converged, 1 = not converged."""
+        t = np.ascontiguousarray(np.broadcast_to(np.asarray(times, dtype=np.float64), (self.n,)))
+        d = np.ascontiguousarray(np.broadcast_to(np.asarray(dts, dtype=np.float64), (self.n,)))
+        it = np.zeros(self.n, dtype=np.int32)
+        st = np.zeros(self.n, dtype=np.int32)
+        eh = np.zeros((self.n, max_iter))
+        rc = self.lib.mafem_batch_time_step(self.h, L.ptr(t, L.f64p), L.ptr(d, L.f64p),
+                                            1 if predict else 0, int(max_iter),
+                                            L.ptr(it, L.i32p), L.ptr(eh, L.f64p),
+                                            L.ptr(st, L.i32p))
+        if rc not in (L.OK, L.NOT_CONVERGED):
+            raise L.LibError(rc, self.lib.mafem_batch_last_error(self.h).decode())
+        return it, [list(eh[k, : it[k]]) for k in range(self.n)], st
+
+    def stats(self):
+        s = L.BatchStats()
+        self.lib.mafem_batch_get_stats(self.h, ctypes.byref(s))
+        return {k: getattr(s, k) for k, _ in L.BatchStats._fields_}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mafem_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def device_for(mesh, p, **args):
     """The DeviceMesh cached on a mesh (rebuilt if the parameters change)."""
     key = (p.motion, p.scenario, p.kb, p.kg, p.zv, p.pn, p.adb, p.am, p.enr,

@@ -37,7 +37,8 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts_match_c(tmp_path):
     src = tmp_path / "lay.c"
     fields = {"mafem_mesh_desc": _lib.MeshDesc, "mafem_params": _lib.ParamsDesc,
-              "mafem_stats": _lib.Stats, "mafem_solver_opts": _lib.SolverOpts}
+              "mafem_stats": _lib.Stats, "mafem_solver_opts": _lib.SolverOpts,
+              "mafem_batch_opts": _lib.BatchOpts, "mafem_batch_stats": _lib.BatchStats}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "mafem.h"', "int main(void){"]
     for cname, cls in fields.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
@@ -66,3 +67,15 @@ def test_create_rejects_null_arguments():
     # calls on a null context fail without touching a device
     assert lib.mafem_predict(None, 1.0) == _lib.EINVAL
     assert lib.mafem_time_step(None, 0.0, 1.0, 14, None, None) == _lib.EINVAL
+
+
+def test_batch_create_rejects_bad_arguments():
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    assert lib.mafem_batch_create(None, 1, None, None) == _lib.EINVAL
+    assert lib.mafem_batch_create(ctypes.byref(h), 0, None, None) == _lib.EINVAL
+    assert not h.value
+    assert b"member" in lib.mafem_batch_last_error(None)
+    assert lib.mafem_batch_time_step(None, None, None, 0, 14, None, None, None) == _lib.EINVAL
+    st = _lib.BatchStats()
+    assert lib.mafem_batch_get_stats(None, ctypes.byref(st)) == _lib.EINVAL
