@@ -94,6 +94,153 @@ def cpu_baseline(p_gpu, mesh, xms, cps, threads):
                        f"{o.mesh.numel / (t1 - t0):.1f}")}
 
 
+def batched_flops(patches):
+    """Algorithmic FLOPs of one factorization of every patch (tile updates)."""
+    tot = 0.0
+    for pt in patches:
+        nt, tb = dev_band_shape(pt.dev)
+        tot += 2.0 * TILE ** 3 * sum(min(tb, nt - 1 - K) ** 2 for K in range(nt))
+    return tot
+
+
+def bench_batched(args):
+    """Batched form: many independent patches (sizes, formulations and
+    parameters drawn per patch) advanced one time step at a time through one
+    batch; FP64 assembly, residual, update and ε test, FP64 GMRES with FP32
+    band-LU preconditioner.  Beside it, a stratified sample of the same
+    patches runs alone on the plain FP64 direct path (one context, one patch
+    at a time) for the same steps: its Newton throughput is the plain
+    per-patch rate, and its final states give the error of the batched run."""
+    import mafem
+    from mafem.patches import Patch, batched_specs
+
+    specs = batched_specs(args.patches, seed=0xC4, sizes=(args.nmin, args.nmax))
+    t0 = time.perf_counter()
+    patches = []
+    for k, sp in enumerate(specs):
+        patches.append(Patch(sp))
+        if (k + 1) % 64 == 0:
+            print(f"bench: {k + 1}/{len(specs)} patches on the device", file=sys.stderr, flush=True)
+    batch = mafem.Batch([pt.dev for pt in patches], precision=args.precision)
+    setup_s = time.perf_counter() - t0
+    dt = 1e-3
+
+    # A patch whose Newton iteration does not converge (the reference raises
+    # AssertionError for it) is a failed simulation: it is reported and left
+    # out of later steps; only converged iterations are counted.
+    alive = np.ones(len(patches), dtype=np.uint8)
+    failed = []
+
+    def step(k):
+        for i, pt in enumerate(patches):
+            if alive[i]:
+                pt.set_pull(pt.spec.speed_at(k))
+        it, eps, st = batch.time_step((k + 1) * dt, dt, predict=True, active=alive)
+        for i in np.where(st == 1)[0]:
+            failed.append({"patch": int(i), "step": k, "n": specs[i].n,
+                           "motion": int(specs[i].motion), "kb": specs[i].kb, "zv": specs[i].zv,
+                           "speed": specs[i].speed, "eps": [float(e) for e in eps[i][:4]]})
+            alive[i] = 0
+            print(f"bench: patch {i} did not converge at step {k} (reported, retired)",
+                  file=sys.stderr, flush=True)
+        return np.where(st == 0, it, 0), batch.stats()
+
+    for k in range(args.warmup):
+        _, st = step(k)
+        print(f"bench: warm-up step {k} done ({st['patch_factors']} factorizations, "
+              f"{st['fp64_patches']} FP64 patches)", file=sys.stderr, flush=True)
+    lat, iters, elems = [], 0, 0
+    agg = {"assemble_us": 0.0, "factor_us": 0.0, "solve_us": 0.0, "update_us": 0.0,
+           "patch_factors": 0, "gmres_iters": 0, "factor_passes": 0, "gmres_cycles": 0,
+           "hard_failures": 0}
+    numel = np.array([pt.mesh.numel for pt in patches])
+    t_start = time.perf_counter()
+    for k in range(args.warmup, args.warmup + args.steps):
+        ts = time.perf_counter()
+        it, st = step(k)
+        lat.append(time.perf_counter() - ts)
+        iters += int(it.sum())
+        elems += int((it * numel).sum())
+        for key in agg:
+            agg[key] += st[key]
+        fp64_patches = st["fp64_patches"]
+        print(f"bench: step {k} {lat[-1]:.3f} s, {int(it.sum())} Newton iterations, "
+              f"{st['patch_factors']} factorizations, {st['gmres_iters']} GMRES steps, "
+              f"{fp64_patches} FP64 patches", file=sys.stderr, flush=True)
+    t_total = time.perf_counter() - t_start
+    states = {}
+    live = [i for i in np.argsort([sp.n for sp in specs], kind="stable") if alive[i]]
+    sample = [int(live[i]) for i in np.linspace(0, len(live) - 1, args.sample).astype(int)]
+    for i in sample:
+        states[i] = patches[i].dev.get_state()
+    fact_flops = batched_flops(patches)
+    batch.close()
+    for pt in patches:
+        pt.close()
+
+    # plain FP64 direct path, one patch at a time, same steps
+    plain_iters, plain_time, errs = 0, 0.0, []
+    for i in sample:
+        pt = Patch(specs[i])
+        pt.dev.set_solver("direct", precision=64, rtol=1e-13)
+        for k in range(args.warmup + args.steps):
+            pt.set_pull(pt.spec.speed_at(k))
+            ts = time.perf_counter()
+            pt.dev.predict(dt)
+            rc, eps = pt.dev.time_step((k + 1) * dt, dt)
+            el = time.perf_counter() - ts
+            if rc != 0:
+                raise RuntimeError(f"plain path: patch {i} did not converge at step {k}")
+            if k >= args.warmup:
+                plain_iters += len(eps)
+                plain_time += el
+        x, c = pt.dev.get_state()
+        xb, cb = states[i]
+        errs.append(max(np.linalg.norm(xb - x) / np.linalg.norm(x),
+                        np.linalg.norm(cb - c) / np.linalg.norm(c)))
+        pt.close()
+    plain_rate = plain_iters / plain_time
+    value = iters / t_total
+    res = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "Newton steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_total / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (FP32 band-LU preconditioner)" if args.precision == 32 else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{len(specs)} independent F_PULL patches, n ~ U{{{args.nmin}.."
+                               f"{args.nmax}}}, kb and zv log-uniform in [0.1, 10], pull ramped "
+                               f"to U[0.2, 3] over 200 steps, motion U{{LAG, ALEVB}}, seed 0xC4, "
+                               f"flat initial state, dt=1e-3, one batch, HIP graph replay",
+                   "patches": len(specs), "elements": int(numel.sum()),
+                   "time_steps": args.steps, "newton_iters": iters},
+        "elem_assemblies_per_s": elems / t_total,
+        "time_steps_per_s": args.steps / t_total,
+        "step_latency_ms_p50": 1e3 * float(np.percentile(lat, 50)),
+        "step_latency_ms_p99": 1e3 * float(np.percentile(lat, 99)),
+        "phase_ms_per_step": {k: agg[k] * 1e-3 / args.steps for k in
+                              ("assemble_us", "factor_us", "solve_us", "update_us")},
+        "patch_factorizations": agg["patch_factors"], "factor_passes": agg["factor_passes"],
+        "gmres_iters": agg["gmres_iters"], "gmres_cycles": agg["gmres_cycles"],
+        "hard_failures": agg["hard_failures"], "fp64_patches": fp64_patches,
+        "failed_patches": failed,
+        "factorization_tflops": (fact_flops / (agg["factor_us"] * 1e-6 / agg["factor_passes"])
+                                 / 1e12) if agg["factor_passes"] else None,
+        "plain_fp64": {"newton_per_s": plain_rate, "sample": len(sample),
+                       "path": "single-context direct FP64, one patch at a time"},
+        "speedup_vs_plain_fp64": value / plain_rate,
+        "rel_err_vs_fp64_max": max(errs),
+        "setup_s": setup_s,
+    }
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,13 +251,25 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--solver", default="direct", choices=["direct", "krylov"])
-    ap.add_argument("--precision", type=int, default=64, choices=[64, 32])
+    ap.add_argument("--precision", type=int, default=None, choices=[64, 32])
+    ap.add_argument("--workload", default="single", choices=["single", "batched"])
+    ap.add_argument("--patches", type=int, default=512)
+    ap.add_argument("--nmin", type=int, default=16)
+    ap.add_argument("--nmax", type=int, default=96)
+    ap.add_argument("--sample", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(independent patches shard across GPUs with no communication; see DESIGN.md)",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batched":
+        if args.precision is None:
+            args.precision = 32
+        bench_batched(args)
+        return
+    if args.precision is None:
+        args.precision = 64
 
     import mafem
     p, mesh, xms, cps = build_case(args.n, args.motion)

@@ -180,7 +180,8 @@ int mafem_get_solver(const mafem_ctx *ctx, mafem_solver_opts *opts);
  * preconditioned by a band LU stored in `factor_precision` (32: the
  * mixed-precision path) and kept across iterations and steps; a patch is
  * refactored when a solve needs more than `refactor_iters` Arnoldi steps or
- * does not converge in `max_cycles` restart cycles.  The batch borrows
+ * does not converge in `max_cycles` restart cycles, and a patch whose fresh
+ * FP32 factorization cannot precondition the solve moves to FP64 storage.  The batch borrows
  * its members: they must outlive it, and while they belong to it only their
  * state calls (set_state, get_state, set_prescribed) may be used on them. */
 typedef struct mafem_batch mafem_batch;
@@ -205,6 +206,8 @@ typedef struct {
   int64_t patch_factors;     /* sum over patches of factorizations */
   int64_t gmres_iters;       /* sum over patches of Arnoldi steps */
   int64_t elements;          /* sum over patches of area elements */
+  int32_t fp64_patches;      /* members whose band LU is stored in FP64 */
+  int32_t reserved;
 } mafem_batch_stats;
 
 /* opts may be NULL (defaults).  All members must live on the current device. */
@@ -212,13 +215,15 @@ int mafem_batch_create(mafem_batch **out, int32_t n, mafem_ctx *const *members,
                        const mafem_batch_opts *opts);
 void mafem_batch_destroy(mafem_batch *b);
 const char *mafem_batch_last_error(const mafem_batch *b);
-/* One backward-Euler step of every member: the predictor update_xms! when
- * `predict` is nonzero (Analysis.jl:70), then Newton.  time and dt hold one
- * value per member; iters[n], eps_hist[n * max_iter] (row per member) and
- * status[n] (0 converged, 1 not converged) may be NULL.  Returns 1 if any
- * member did not converge. */
-int mafem_batch_time_step(mafem_batch *b, const double *time, const double *dt, int32_t predict,
-                          int32_t max_iter, int32_t *iters, double *eps_hist, int32_t *status);
+/* One backward-Euler step of the members selected by `active` (NULL: all):
+ * the predictor update_xms! when `predict` is nonzero (Analysis.jl:70), then
+ * Newton.  time and dt hold one value per member; iters[n], eps_hist[n *
+ * max_iter] (row per member) and status[n] (0 converged, 1 not converged,
+ * 2 not selected) may be NULL.  Members not selected are left untouched.
+ * Returns 1 if any selected member did not converge. */
+int mafem_batch_time_step(mafem_batch *b, const double *time, const double *dt,
+                          const uint8_t *active, int32_t predict, int32_t max_iter, int32_t *iters,
+                          double *eps_hist, int32_t *status);
 int mafem_batch_get_stats(const mafem_batch *b, mafem_batch_stats *out);
 
 #ifdef __cplusplus

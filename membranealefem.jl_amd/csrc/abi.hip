@@ -49,7 +49,7 @@ void free_all(Ctx &c) {
                   c.d_rsrc, c.d_r,      c.d_x,       c.d_w,       c.d_t,      c.d_b,
                   c.d_red,  c.d_csc_src, c.d_csc_val, c.d_eq,     c.d_band,   c.d_linv,
                   c.d_uinv, c.d_piv,  c.d_V,       c.d_Z,       c.d_h,      c.d_coef,
-                  c.d_mdot_part, c.d_t2};
+                  c.d_mdot_part, c.d_t2, c.d_presc_idx, c.d_presc_val};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c.h_red) (void)hipHostFree(c.h_red);
@@ -543,14 +543,24 @@ int mafem_set_prescribed(mafem_ctx *ctx, int64_t n, const int64_t *node, const i
       idx[k] = (node[k] - 1) + c.numnp * (dof[k] - 1);
     }
     if (n == 0) return MAFEM_OK;
-    int64_t *d_idx = dupload(c, idx);
-    double *d_val = dalloc<double>(c, n);
-    MAFEM_HIP(hipMemcpy(d_val, val, sizeof(double) * n, hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_scatter_idx, dim3(nb256(n)), dim3(256), 0, c.stream, n, d_idx, d_val, c.d_cps);
+    if (n > c.presc_cap) {  // staging buffers grow on demand and are kept
+      if (c.d_presc_idx) (void)hipFree(c.d_presc_idx);
+      if (c.d_presc_val) (void)hipFree(c.d_presc_val);
+      c.d_presc_idx = nullptr;
+      c.d_presc_val = nullptr;
+      c.presc_cap = 0;
+      c.d_presc_idx = dalloc<int64_t>(c, n);
+      c.d_presc_val = dalloc<double>(c, n);
+      c.presc_cap = n;
+    }
+    MAFEM_HIP(hipMemcpyAsync(c.d_presc_idx, idx.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice,
+                             c.stream));
+    MAFEM_HIP(hipMemcpyAsync(c.d_presc_val, val, sizeof(double) * n, hipMemcpyHostToDevice,
+                             c.stream));
+    hipLaunchKernelGGL(k_scatter_idx, dim3(nb256(n)), dim3(256), 0, c.stream, n, c.d_presc_idx,
+                       c.d_presc_val, c.d_cps);
     MAFEM_HIP(hipGetLastError());
     MAFEM_HIP(hipStreamSynchronize(c.stream));
-    (void)hipFree(d_idx);
-    (void)hipFree(d_val);
     return MAFEM_OK;
   });
 }

@@ -518,14 +518,21 @@ __device__ __forceinline__ bool fac_on(const uint8_t *on, const uint8_t *fac, in
   return on[p] && fac[p];
 }
 
+// storage precision a template instance serves; patches of the other
+// precision skip the launch
+template <class S>
+constexpr int prec_of() {
+  return sizeof(S) == 4 ? 32 : 64;
+}
+
 // 64 KiB chunks of a patch's band storage, 16 bytes per store
 __global__ __launch_bounds__(256) void kb_zero_band(Seg sg, const PatchDev *__restrict__ pd,
                                                     const uint8_t *__restrict__ on,
-                                                    const uint8_t *__restrict__ fac, int es) {
+                                                    const uint8_t *__restrict__ fac) {
   const int p = sg.map[blockIdx.x];
   if (!fac_on(on, fac, p)) return;
   const PatchDev &d = pd[p];
-  const int64_t bytes = d.nt * (2 * d.tb + 1) * (int64_t)T * T * es;
+  const int64_t bytes = d.nt * (2 * d.tb + 1) * (int64_t)T * T * (d.prec / 8);
   const int64_t base = (int64_t)(blockIdx.x - sg.blk0[p]) * 65536;
   uint4 *dst = reinterpret_cast<uint4 *>(d.band);
 #pragma unroll
@@ -540,7 +547,7 @@ __global__ __launch_bounds__(256) void kb_bsr_to_band(Seg sg, const PatchDev *__
                                                       const uint8_t *__restrict__ on,
                                                       const uint8_t *__restrict__ fac) {
   const int p = sg.map[blockIdx.x];
-  if (!fac_on(on, fac, p)) return;
+  if (!fac_on(on, fac, p) || pd[p].prec != prec_of<S>()) return;
   const PatchDev &d = pd[p];
   bsr_to_band_body<S>(blockIdx.x - sg.blk0[p], d.nnzb, d.ndf, d.tb, d.brow, d.colidx, d.bsr,
                       (S *)d.band);
@@ -552,7 +559,7 @@ __global__ __launch_bounds__(64) void kb_band_pad(const PatchDev *__restrict__ p
                                                   const uint8_t *__restrict__ on,
                                                   const uint8_t *__restrict__ fac) {
   const int p = blockIdx.x;
-  if (!fac_on(on, fac, p)) return;
+  if (!fac_on(on, fac, p) || pd[p].prec != prec_of<S>()) return;
   const PatchDev &d = pd[p];
   const int64_t R = d.nx + threadIdx.x;
   if (R < d.nt * T) tile_ptr((S *)d.band, d.tb, R / T, R / T)[(R % T) * T + (R % T)] = S(1);
@@ -580,7 +587,7 @@ __global__ __launch_bounds__(64) void kb_diag(StepMap sm, int64_t K, const int32
                                               const uint8_t *__restrict__ fac) {
   int local, tb;
   const int p = step_locate<0>(sm, order, local, tb);
-  if (!fac_on(on, fac, p)) return;
+  if (!fac_on(on, fac, p) || pd[p].prec != prec_of<S>()) return;
   diag_body<S>(K, tb, (S *)pd[p].band, pd[p].piv);
 }
 
@@ -591,7 +598,7 @@ __global__ __launch_bounds__(64) void kb_panel(StepMap sm, int64_t K, const int3
                                                const uint8_t *__restrict__ fac) {
   int local, tb;
   const int p = step_locate<1>(sm, order, local, tb);
-  if (!fac_on(on, fac, p)) return;
+  if (!fac_on(on, fac, p) || pd[p].prec != prec_of<S>()) return;
   panel_body<S>(local, K, pd[p].nt, tb, (S *)pd[p].band, pd[p].piv);
 }
 
@@ -602,7 +609,7 @@ __global__ __launch_bounds__(256) void kb_update(StepMap sm, int64_t K, const in
                                                  const uint8_t *__restrict__ fac) {
   int local, tb;
   const int p = step_locate<2>(sm, order, local, tb);
-  if (!fac_on(on, fac, p)) return;
+  if (!fac_on(on, fac, p) || pd[p].prec != prec_of<S>()) return;
   update_body<S>(local, K, pd[p].nt, tb, (S *)pd[p].band);
 }
 
@@ -611,7 +618,7 @@ __global__ __launch_bounds__(64) void kb_tile_inv(Seg sg, const PatchDev *__rest
                                                   const uint8_t *__restrict__ on,
                                                   const uint8_t *__restrict__ fac) {
   const int p = sg.map[blockIdx.x];
-  if (!fac_on(on, fac, p)) return;
+  if (!fac_on(on, fac, p) || pd[p].prec != prec_of<S>()) return;
   const PatchDev &d = pd[p];
   tile_inv_body<S>(blockIdx.x - sg.blk0[p], d.tb, (const S *)d.band, (S *)d.linv, (S *)d.uinv);
 }
@@ -621,7 +628,7 @@ __global__ __launch_bounds__(256) void kb_fold(Seg sg, const PatchDev *__restric
                                                const uint8_t *__restrict__ on,
                                                const uint8_t *__restrict__ fac) {
   const int p = sg.map[blockIdx.x];
-  if (!fac_on(on, fac, p)) return;
+  if (!fac_on(on, fac, p) || pd[p].prec != prec_of<S>()) return;
   const PatchDev &d = pd[p];
   fold_body<S>(blockIdx.x - sg.blk0[p], d.nt, d.tb, (S *)d.band, (const S *)d.linv,
                (const S *)d.uinv);
@@ -632,7 +639,7 @@ template <class S>
 __global__ __launch_bounds__(256) void kb_diag_apply(Seg sg, const PatchDev *__restrict__ pd,
                                                      const uint8_t *__restrict__ mask, int which) {
   const int p = sg.map[blockIdx.x];
-  if (!mask[p]) return;
+  if (!mask[p] || pd[p].prec != prec_of<S>()) return;
   const PatchDev &d = pd[p];
   if (which == 0)
     diag_apply_body<S>(blockIdx.x - sg.blk0[p], (const S *)d.linv, d.piv, d.w, d.t2);
@@ -645,7 +652,7 @@ template <class S>
 __global__ __launch_bounds__(SW) void kb_fwd_sweep(const PatchDev *__restrict__ pd,
                                                    const uint8_t *__restrict__ mask) {
   const int p = blockIdx.x;
-  if (!mask[p]) return;
+  if (!mask[p] || pd[p].prec != prec_of<S>()) return;
   fwd_sweep_body<S>(pd[p].nt, pd[p].tb, (const S *)pd[p].band, pd[p].t2);
 }
 
@@ -653,7 +660,7 @@ template <class S>
 __global__ __launch_bounds__(SW) void kb_bwd_sweep(const PatchDev *__restrict__ pd,
                                                    const uint8_t *__restrict__ mask) {
   const int p = blockIdx.x;
-  if (!mask[p]) return;
+  if (!mask[p] || pd[p].prec != prec_of<S>()) return;
   bwd_sweep_body<S>(pd[p].nt, pd[p].tb, (const S *)pd[p].band, pd[p].t);
 }
 
@@ -763,22 +770,15 @@ void band_solve(Ctx &c, const double *d_rhs, double *d_x) {
 }
 
 template <class S>
-static void b_factor_t(BatchPlan &B) {
+static void b_factor_t(BatchPlan &B, const std::vector<std::vector<StepMap>> &steps) {
   const BatchFlags &f = B.f;
-  hipLaunchKernelGGL(kb_zero_band, dim3(B.zero.nblk), dim3(256), 0, B.stream, B.zero, B.d_pd, f.on,
-                     f.fac, (int)sizeof(S));
   hipLaunchKernelGGL(kb_bsr_to_band<S>, dim3(B.bsr.nblk), dim3(256), 0, B.stream, B.bsr, B.d_pd,
                      f.on, f.fac);
   hipLaunchKernelGGL(kb_band_pad<S>, dim3(B.P), dim3(64), 0, B.stream, B.d_pd, f.on, f.fac);
   MAFEM_HIP(hipGetLastError());
-  for (int64_t K = 0; K < (int64_t)B.steps.size(); ++K)
-    for (const StepMap &sm : B.steps[K]) {
+  for (int64_t K = 0; K < (int64_t)steps.size(); ++K)
+    for (const StepMap &sm : steps[K]) {
       const int32_t nd = sm.blk0[sm.ncls];
-      int32_t np = 0, nu = 0;
-      for (int c = 0; c < sm.ncls; ++c) {
-        np += sm.count[c] * 3 * sm.tb[c];
-        nu += sm.count[c] * sm.tb[c] * sm.tb[c];
-      }
       StepMap smp = sm, smu = sm;
       int32_t bp = 0, bu = 0;
       for (int c = 0; c < sm.ncls; ++c) {
@@ -791,10 +791,10 @@ static void b_factor_t(BatchPlan &B) {
       smu.blk0[sm.ncls] = bu;
       hipLaunchKernelGGL(kb_diag<S>, dim3(nd), dim3(64), 0, B.stream, sm, K, B.d_order, B.d_pd,
                          f.on, f.fac);
-      hipLaunchKernelGGL(kb_panel<S>, dim3(np), dim3(64), 0, B.stream, smp, K, B.d_order, B.d_pd,
+      hipLaunchKernelGGL(kb_panel<S>, dim3(bp), dim3(64), 0, B.stream, smp, K, B.d_order, B.d_pd,
                          f.on, f.fac);
-      if (nu > 0)
-        hipLaunchKernelGGL(kb_update<S>, dim3(nu), dim3(256), 0, B.stream, smu, K, B.d_order,
+      if (bu > 0)
+        hipLaunchKernelGGL(kb_update<S>, dim3(bu), dim3(256), 0, B.stream, smu, K, B.d_order,
                            B.d_pd, f.on, f.fac);
     }
   hipLaunchKernelGGL(kb_tile_inv<S>, dim3(B.tiles.nblk), dim3(64), 0, B.stream, B.tiles, B.d_pd,
@@ -805,10 +805,11 @@ static void b_factor_t(BatchPlan &B) {
 }
 
 void b_factor(BatchPlan &B) {
-  if (B.prec == 32)
-    b_factor_t<float>(B);
-  else
-    b_factor_t<double>(B);
+  hipLaunchKernelGGL(kb_zero_band, dim3(B.zero.nblk), dim3(256), 0, B.stream, B.zero, B.d_pd,
+                     B.f.on, B.f.fac);
+  MAFEM_HIP(hipGetLastError());
+  if (B.has32) b_factor_t<float>(B, B.steps32);
+  if (B.has64) b_factor_t<double>(B, B.steps64);
 }
 
 template <class S>
@@ -823,10 +824,8 @@ static void b_band_solve_t(BatchPlan &B, const uint8_t *mask) {
 }
 
 void b_band_solve(BatchPlan &B, const uint8_t *mask) {
-  if (B.prec == 32)
-    b_band_solve_t<float>(B, mask);
-  else
-    b_band_solve_t<double>(B, mask);
+  if (B.has32) b_band_solve_t<float>(B, mask);
+  if (B.has64) b_band_solve_t<double>(B, mask);
 }
 
 }  // namespace mafem

@@ -30,7 +30,9 @@ struct mafem_batch {
   int32_t *h_count = nullptr;  // pinned mirror of B.f.count
   hipGraphExec_t g_asm = nullptr, g_fac = nullptr, g_cyc = nullptr, g_end = nullptr,
                  g_upd = nullptr;
-  std::vector<void *> owned;   // device allocations of the plan
+  std::vector<void *> owned;   // device allocations of the plan (rebuilt with it)
+  std::vector<void *> fowned;  // device allocations of the flags (live with the batch)
+  std::vector<int> prec;       // band storage precision of each member
   mafem_batch_stats stats{};
 };
 
@@ -50,6 +52,18 @@ T *balloc(mafem_batch &b, int64_t n) {
     throw BatchError{"device allocation failed", MAFEM_ENOMEM};
   }
   b.owned.push_back(p);
+  return (T *)p;
+}
+
+template <class T>
+T *falloc(mafem_batch &b, int64_t n) {
+  void *p = nullptr;
+  if (n <= 0) n = 1;
+  if (hipMalloc(&p, sizeof(T) * (size_t)n) != hipSuccess) {
+    (void)hipGetLastError();
+    throw BatchError{"device allocation failed", MAFEM_ENOMEM};
+  }
+  b.fowned.push_back(p);
   return (T *)p;
 }
 
@@ -105,27 +119,73 @@ PatchDev describe(const Ctx &c) {
   d.gptr = c.d_gptr; d.gsrc = c.d_gsrc; d.rptr = c.d_rptr; d.rsrc = c.d_rsrc;
   d.r = c.d_r; d.b = c.d_b; d.x = c.d_x; d.w = c.d_w; d.t = c.d_t; d.t2 = c.d_t2;
   d.band = c.d_band; d.linv = c.d_linv; d.uinv = c.d_uinv; d.piv = c.d_piv;
+  d.prec = c.band_prec;
   return d;
 }
 
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
-// Builds descriptors, segment maps, factorization step tables and flags.
+// Factorization step tables of the patches of one precision group.
+std::vector<std::vector<StepMap>> step_tables(const mafem_batch &b, const std::vector<int32_t> &order,
+                                              int prec) {
+  struct Cls { int first, n, tb; };
+  std::vector<Cls> cls;
+  int64_t max_nt = 0;
+  for (int i = 0; i < (int)order.size(); ++i) {
+    const PatchDev &d = b.h_pd[order[i]];
+    if (d.prec != prec) continue;
+    if (cls.empty() || cls.back().tb != d.tb || cls.back().first + cls.back().n != i)
+      cls.push_back({i, 0, d.tb});
+    cls.back().n++;
+    max_nt = std::max(max_nt, d.nt);
+  }
+  std::vector<std::vector<StepMap>> steps(max_nt);
+  for (int64_t K = 0; K < max_nt; ++K) {
+    StepMap sm{};
+    int32_t blk = 0;
+    for (const Cls &c : cls) {
+      int cnt = 0;
+      while (cnt < c.n && b.h_pd[order[c.first + cnt]].nt > K) ++cnt;
+      if (cnt == 0) continue;
+      if (sm.ncls == MAXC) {
+        sm.blk0[sm.ncls] = blk;
+        steps[K].push_back(sm);
+        sm = StepMap{};
+        blk = 0;
+      }
+      sm.first[sm.ncls] = c.first;
+      sm.count[sm.ncls] = cnt;
+      sm.tb[sm.ncls] = c.tb;
+      sm.blk0[sm.ncls] = blk;
+      blk += cnt;
+      sm.ncls++;
+    }
+    if (sm.ncls > 0) {
+      sm.blk0[sm.ncls] = blk;
+      steps[K].push_back(sm);
+    }
+  }
+  return steps;
+}
+
+// Builds descriptors, segment maps and factorization step tables (the flags
+// are allocated once per batch and keep their contents).
 void build_plan(mafem_batch &b) {
   free_plan(b);
   BatchPlan &B = b.B;
   const int P = (int)b.members.size();
   B.P = P;
-  B.prec = b.opts.factor_precision;
   B.m = b.opts.restart;
   B.refactor_iters = b.opts.refactor_iters;
   B.rtol = b.opts.rtol;
   b.h_pd.resize(P);
+  B.has32 = B.has64 = false;
   for (int p = 0; p < P; ++p) {
     Ctx &c = b.members[p]->c;
-    band_alloc(c, B.prec);
+    band_alloc(c, b.prec[p]);
     b.h_pd[p] = describe(c);
     b.h_pd[p].V = balloc<double>(b, (B.m + 1) * c.nt * (int64_t)TILE);
+    (b.prec[p] == 32 ? B.has32 : B.has64) = true;
   }
   B.d_pd = bupload(b, b.h_pd);
 
@@ -150,11 +210,10 @@ void build_plan(mafem_batch &b) {
   std::vector<int64_t> bb(P), bv(P), bn(P), bz(P), bt(P), bf(P);
   for (int p = 0; p < P; ++p) {
     const PatchDev &d = b.h_pd[p];
-    const int64_t es = B.prec == 32 ? 4 : 8;
     bb[p] = cdiv(d.nnzb * d.ndf * d.ndf, 256);
     bv[p] = cdiv(d.nt * TILE, 256);
     bn[p] = cdiv(d.numnp, 256);
-    bz[p] = cdiv(d.nt * (2 * d.tb + 1) * (int64_t)TILE * TILE * es, 65536);
+    bz[p] = cdiv(d.nt * (2 * d.tb + 1) * (int64_t)TILE * TILE * (d.prec / 8), 65536);
     bt[p] = d.nt;
     bf[p] = d.nt * (2 * d.tb + 1);
   }
@@ -165,86 +224,60 @@ void build_plan(mafem_batch &b) {
   B.tiles = make_seg(b, bt);
   B.fold = make_seg(b, bf);
 
-  // factorization order: classes of equal tb, each by decreasing nt
+  // factorization order: by precision, then classes of equal tb, each by
+  // decreasing nt
   std::vector<int32_t> order(P);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
     const PatchDev &a = b.h_pd[x], &c = b.h_pd[y];
+    if (a.prec != c.prec) return a.prec < c.prec;
     return a.tb != c.tb ? a.tb < c.tb : a.nt > c.nt;
   });
   B.d_order = bupload(b, order);
-  struct Cls { int first, n, tb; };
-  std::vector<Cls> cls;
-  for (int i = 0; i < P; ++i) {
-    const int tb = b.h_pd[order[i]].tb;
-    if (cls.empty() || cls.back().tb != tb) cls.push_back({i, 0, tb});
-    cls.back().n++;
-  }
+  B.steps32 = step_tables(b, order, 32);
+  B.steps64 = step_tables(b, order, 64);
   B.max_nt = 0;
   for (const PatchDev &d : b.h_pd) B.max_nt = std::max(B.max_nt, d.nt);
-  B.steps.assign(B.max_nt, {});
-  for (int64_t K = 0; K < B.max_nt; ++K) {
-    StepMap sm{};
-    int32_t blk = 0;
-    for (const Cls &c : cls) {
-      int cnt = 0;
-      while (cnt < c.n && b.h_pd[order[c.first + cnt]].nt > K) ++cnt;
-      if (cnt == 0) continue;
-      if (sm.ncls == MAXC) {
-        sm.blk0[sm.ncls] = blk;
-        B.steps[K].push_back(sm);
-        sm = StepMap{};
-        blk = 0;
-      }
-      sm.first[sm.ncls] = c.first;
-      sm.count[sm.ncls] = cnt;
-      sm.tb[sm.ncls] = c.tb;
-      sm.blk0[sm.ncls] = blk;
-      blk += cnt;
-      sm.ncls++;
-    }
-    if (sm.ncls > 0) {
-      sm.blk0[sm.ncls] = blk;
-      B.steps[K].push_back(sm);
-    }
-  }
+  B.f.part = balloc<double>(b, std::max<int64_t>(B.vec.nblk, 1));
+  B.f.part2 = balloc<double>(b, std::max<int64_t>((int64_t)B.vec.nblk * (B.m + 1), 1));
+}
 
-  // flags
-  BatchFlags &f = B.f;
-  const int m = B.m;
-  f.on = balloc<uint8_t>(b, P);
-  f.sol = balloc<uint8_t>(b, P);
-  f.fac = balloc<uint8_t>(b, P);
-  f.kon = balloc<uint8_t>(b, P);
-  f.fresh = balloc<uint8_t>(b, P);
-  f.iters = balloc<int32_t>(b, P);
-  f.kits = balloc<int32_t>(b, P);
-  f.kdim = balloc<int32_t>(b, P);
-  f.nfac = balloc<int32_t>(b, P);
-  f.kits_tot = balloc<int32_t>(b, P);
-  f.time = balloc<double>(b, P);
-  f.dt = balloc<double>(b, P);
-  f.bn = balloc<double>(b, P);
-  f.relres = balloc<double>(b, P);
-  f.eps = balloc<double>(b, P);
-  f.scale = balloc<double>(b, P);
-  f.H = balloc<double>(b, (int64_t)P * m * (m + 1));
-  f.cs = balloc<double>(b, (int64_t)P * m);
-  f.sn = balloc<double>(b, (int64_t)P * m);
-  f.g = balloc<double>(b, (int64_t)P * (m + 1));
-  f.hc = balloc<double>(b, (int64_t)P * (m + 1));
+// Per-patch flags and Arnoldi state, once per batch.
+void alloc_flags(mafem_batch &b) {
+  BatchFlags &f = b.B.f;
+  const int P = (int)b.members.size(), m = b.opts.restart;
+  f.on = falloc<uint8_t>(b, P);
+  f.sol = falloc<uint8_t>(b, P);
+  f.fac = falloc<uint8_t>(b, P);
+  f.kon = falloc<uint8_t>(b, P);
+  f.fresh = falloc<uint8_t>(b, P);
+  f.esc = falloc<uint8_t>(b, P);
+  f.iters = falloc<int32_t>(b, P);
+  f.kits = falloc<int32_t>(b, P);
+  f.kdim = falloc<int32_t>(b, P);
+  f.nfac = falloc<int32_t>(b, P);
+  f.kits_tot = falloc<int32_t>(b, P);
+  f.time = falloc<double>(b, P);
+  f.dt = falloc<double>(b, P);
+  f.bn = falloc<double>(b, P);
+  f.relres = falloc<double>(b, P);
+  f.eps = falloc<double>(b, P);
+  f.scale = falloc<double>(b, P);
+  f.H = falloc<double>(b, (int64_t)P * m * (m + 1));
+  f.cs = falloc<double>(b, (int64_t)P * m);
+  f.sn = falloc<double>(b, (int64_t)P * m);
+  f.g = falloc<double>(b, (int64_t)P * (m + 1));
+  f.hc = falloc<double>(b, (int64_t)P * (m + 1));
   f.eps_hist = nullptr;
-  f.part = balloc<double>(b, std::max<int64_t>(B.vec.nblk, 1));
-  f.part2 = balloc<double>(b, std::max<int64_t>((int64_t)B.vec.nblk * (m + 1), 1));
-  f.count = balloc<int32_t>(b, 4);
+  f.count = falloc<int32_t>(b, 8);
   const std::vector<uint8_t> zeros8(P, 0), ones8(P, 1);
-  const std::vector<int32_t> zeros32(P, 0);
-  for (uint8_t *a : {f.on, f.sol, f.kon, f.fresh})
+  const std::vector<int32_t> zeros32(std::max(P, 8), 0);
+  for (uint8_t *a : {f.on, f.sol, f.kon, f.fresh, f.esc})
     MAFEM_HIP(hipMemcpy(a, zeros8.data(), P, hipMemcpyHostToDevice));
   MAFEM_HIP(hipMemcpy(f.fac, ones8.data(), P, hipMemcpyHostToDevice));  // no factor yet
   for (int32_t *a : {f.iters, f.kits, f.kdim, f.nfac, f.kits_tot})
     MAFEM_HIP(hipMemcpy(a, zeros32.data(), 4 * P, hipMemcpyHostToDevice));
-  MAFEM_HIP(hipMemcpy(f.count, zeros32.data(), 16, hipMemcpyHostToDevice));
+  MAFEM_HIP(hipMemcpy(f.count, zeros32.data(), 32, hipMemcpyHostToDevice));
 }
 
 // eps_hist depends on max_iter: (re)allocated when it changes
@@ -253,7 +286,7 @@ void ensure_hist(mafem_batch &b, int max_iter) {
   if (B.f.eps_hist && B.max_iter == max_iter) return;
   destroy_graphs(b);  // the update graph holds the old pointer
   B.max_iter = max_iter;
-  B.f.eps_hist = balloc<double>(b, (int64_t)B.P * max_iter);
+  B.f.eps_hist = falloc<double>(b, (int64_t)B.P * max_iter);
 }
 
 // The plan is stale when a member's band storage moved (a precision change
@@ -261,7 +294,7 @@ void ensure_hist(mafem_batch &b, int max_iter) {
 bool plan_stale(const mafem_batch &b) {
   for (size_t p = 0; p < b.members.size(); ++p) {
     const Ctx &c = b.members[p]->c;
-    if (c.d_band != b.h_pd[p].band || c.band_prec != b.B.prec) return true;
+    if (c.d_band != b.h_pd[p].band || c.band_prec != b.prec[p]) return true;
   }
   return false;
 }
@@ -306,8 +339,22 @@ void run(mafem_batch &b, hipGraphExec_t &g, void (*issue)(mafem_batch &)) {
   MAFEM_HIP(hipGraphLaunch(g, b.B.stream));
 }
 
+// Patches whose fresh FP32 factorization could not precondition the solve
+// move to FP64 band storage: the plan is rebuilt around them (flags and
+// iterates are kept) and they are refactored before the solve continues.
+void escalate(mafem_batch &b) {
+  const int P = b.B.P;
+  std::vector<uint8_t> esc(P);
+  MAFEM_HIP(hipMemcpy(esc.data(), b.B.f.esc, P, hipMemcpyDeviceToHost));
+  for (int p = 0; p < P; ++p)
+    if (esc[p]) b.prec[p] = 64;
+  const std::vector<uint8_t> zeros8(P, 0);
+  MAFEM_HIP(hipMemcpy(b.B.f.esc, zeros8.data(), P, hipMemcpyHostToDevice));
+  build_plan(b);
+}
+
 void read_counts(mafem_batch &b) {
-  MAFEM_HIP(hipMemcpyAsync(b.h_count, b.B.f.count, 16, hipMemcpyDeviceToHost, b.B.stream));
+  MAFEM_HIP(hipMemcpyAsync(b.h_count, b.B.f.count, 32, hipMemcpyDeviceToHost, b.B.stream));
   MAFEM_HIP(hipStreamSynchronize(b.B.stream));
 }
 
@@ -377,7 +424,9 @@ int mafem_batch_create(mafem_batch **out, int32_t n, mafem_ctx *const *members,
       }
       MAFEM_HIP(hipStreamCreateWithFlags(&b.B.stream, hipStreamNonBlocking));
       for (auto &e : b.ev) MAFEM_HIP(hipEventCreate(&e));
-      MAFEM_HIP(hipHostMalloc((void **)&b.h_count, 16, hipHostMallocDefault));
+      MAFEM_HIP(hipHostMalloc((void **)&b.h_count, 32, hipHostMallocDefault));
+      b.prec.assign(n, o.factor_precision);
+      alloc_flags(b);
       build_plan(b);
     } catch (const HipError &h) {
       b.err = std::string("HIP error ") + hipGetErrorString(h.code) + " in " + h.what;
@@ -404,6 +453,7 @@ void mafem_batch_destroy(mafem_batch *bp) {
   (void)hipSetDevice(bp->dev);
   if (bp->B.stream) (void)hipStreamSynchronize(bp->B.stream);
   free_plan(*bp);
+  for (void *p : bp->fowned) (void)hipFree(p);
   for (auto &e : bp->ev)
     if (e) (void)hipEventDestroy(e);
   if (bp->h_count) (void)hipHostFree(bp->h_count);
@@ -415,8 +465,9 @@ const char *mafem_batch_last_error(const mafem_batch *b) {
   return b ? b->err.c_str() : g_batch_err.c_str();
 }
 
-int mafem_batch_time_step(mafem_batch *bp, const double *time, const double *dt, int32_t predict,
-                          int32_t max_iter, int32_t *iters, double *eps_hist, int32_t *status) {
+int mafem_batch_time_step(mafem_batch *bp, const double *time, const double *dt,
+                          const uint8_t *active, int32_t predict, int32_t max_iter, int32_t *iters,
+                          double *eps_hist, int32_t *status) {
   BATCH_GUARD(bp, {
     if (!time || !dt) {
       b.err = "null time or dt";
@@ -433,7 +484,9 @@ int mafem_batch_time_step(mafem_batch *bp, const double *time, const double *dt,
     if (plan_stale(b)) build_plan(b);
     ensure_hist(b, max_iter);
     BatchFlags &f = B.f;
-    const std::vector<uint8_t> ones8(P, 1);
+    std::vector<uint8_t> ones8(P, 1);
+    if (active)
+      for (int p = 0; p < P; ++p) ones8[p] = active[p] ? 1 : 0;
     const std::vector<int32_t> zeros32(P, 0);
     MAFEM_HIP(hipMemcpyAsync(f.time, time, sizeof(double) * P, hipMemcpyHostToDevice, B.stream));
     MAFEM_HIP(hipMemcpyAsync(f.dt, dt, sizeof(double) * P, hipMemcpyHostToDevice, B.stream));
@@ -458,7 +511,7 @@ int mafem_batch_time_step(mafem_batch *bp, const double *time, const double *dt,
       MAFEM_HIP(hipEventRecord(b.ev[1], B.stream));
       run(b, b.g_asm, issue_asm);
       MAFEM_HIP(hipEventRecord(b.ev[2], B.stream));
-      for (int attempt = 0; attempt < 2; ++attempt) {
+      for (int attempt = 0; attempt < 3; ++attempt) {
         if (pend > 0) {
           MAFEM_HIP(hipEventRecord(b.ev[3], B.stream));
           run(b, b.g_fac, issue_fac);
@@ -481,6 +534,7 @@ int mafem_batch_time_step(mafem_batch *bp, const double *time, const double *dt,
         run(b, b.g_end, issue_end);
         read_counts(b);
         pend = b.h_count[2];
+        if (b.h_count[4] > 0) escalate(b);
         if (b.h_count[1] == 0) break;
       }
       MAFEM_HIP(hipEventRecord(b.ev[3], B.stream));
@@ -514,11 +568,18 @@ int mafem_batch_time_step(mafem_batch *bp, const double *time, const double *dt,
                           hipMemcpyDeviceToHost));
     read_counts(b);
     st.hard_failures = b.h_count[3];
+    for (int p = 0; p < P; ++p) st.fp64_patches += b.prec[p] == 64 ? 1 : 0;
     int any_fail = 0;
     for (int p = 0; p < P; ++p) {
+      const bool sel = ones8[p] != 0;
       const bool ok = h_eps[p] < b.h_pd[p].enr && h_it[p] > 0;
-      if (iters) iters[p] = h_it[p];
-      if (status) status[p] = ok ? 0 : 1;
+      if (iters) iters[p] = sel ? h_it[p] : 0;
+      if (status) status[p] = !sel ? 2 : (ok ? 0 : 1);
+      if (!sel) {
+        if (eps_hist)
+          for (int k = 0; k < max_iter; ++k) eps_hist[(size_t)p * max_iter + k] = 0.0;
+        continue;
+      }
       if (!ok) any_fail = 1;
       st.newton_iters = std::max(st.newton_iters, h_it[p]);
       st.patch_iters += h_it[p];

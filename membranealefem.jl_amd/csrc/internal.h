@@ -121,6 +121,11 @@ struct Ctx {
 
   mafem_stats stats{};
   int32_t n_band_solve = 0;     // running count of triangular-solve sweeps
+
+  // staging of mafem_set_prescribed (grown on demand, kept)
+  int64_t *d_presc_idx = nullptr;
+  double *d_presc_val = nullptr;
+  int64_t presc_cap = 0;
 };
 
 // ---------------------------------------------------------------- batches
@@ -151,6 +156,8 @@ struct PatchDev {
   void *band, *linv, *uinv;
   int32_t *piv;
   double *V;         // GMRES basis, m + 1 vectors of length nt TILE (batch-owned)
+  int32_t prec;      // band storage precision of this patch: 32 or 64
+  int32_t pad1;
 };
 
 // Block -> patch map of one batched launch.
@@ -197,12 +204,14 @@ struct BatchFlags {
   double *eps_hist;  // [P][max_iter]
   double *part;      // [vec blocks] per-block partial sums
   double *part2;     // [m+1][vec blocks] per-block partial dot products
-  int32_t *count;    // [4] patches still on, solves unconverged, factorizations pending, hard failures
+  uint8_t *esc;      // FP32 factorization could not precondition the solve: move to FP64
+  int32_t *count;    // [8] patches still on, solves unconverged, factorizations pending,
+                     // hard failures, precision escalations
 };
 
 struct BatchPlan {
   int P = 0;
-  int prec = 32;
+  bool has32 = false, has64 = false;  // precision groups present
   int max_iter = 14;
   int m = 20;                // GMRES basis size
   int refactor_iters = 12;
@@ -214,7 +223,7 @@ struct BatchPlan {
   std::vector<int> motions;         // motions present
   std::vector<Seg> area, edge;      // per motion present
   Seg bsr, vec, node, zero, tiles, fold;
-  std::vector<std::vector<StepMap>> steps;  // launches of each factorization step
+  std::vector<std::vector<StepMap>> steps32, steps64;  // factorization steps per precision group
   int64_t max_nt = 0;
 };
 

@@ -530,29 +530,38 @@ __global__ __launch_bounds__(256) void kb_count_sol(int P, BatchFlags f) {
 }
 
 // Unconverged solves after the cycles: with a stale factorization, refactor
-// and continue from the current iterate; with a fresh one, keep the inexact
-// correction and count a hard failure.  count[1] = solves to continue,
-// count[2] = factorizations pending.
-__global__ __launch_bounds__(256) void kb_solve_end(int P, BatchFlags f) {
-  int redo = 0, pend = 0, hard = 0;
+// and continue from the current iterate; with a fresh FP32 one, move the
+// patch to FP64 storage (esc) and continue after refactoring; with a fresh
+// FP64 one, keep the inexact correction and count a hard failure.
+// count[1] = solves to continue, count[2] = factorizations pending,
+// count[4] = escalations.
+__global__ __launch_bounds__(256) void kb_solve_end(int P, const PatchDev *__restrict__ pd,
+                                                    BatchFlags f) {
+  int redo = 0, pend = 0, hard = 0, esc = 0;
   for (int p = threadIdx.x; p < P; p += 256) {
     if (f.sol[p]) {
-      if (f.fresh[p]) {
-        f.sol[p] = 0;
-        ++hard;
-      } else {
+      if (!f.fresh[p]) {
         f.fac[p] = 1;
         ++redo;
+      } else if (pd[p].prec == 32) {
+        f.esc[p] = 1;
+        f.fac[p] = 1;
+        ++redo;
+        ++esc;
+      } else {
+        f.sol[p] = 0;
+        ++hard;
       }
     }
     pend += f.on[p] && f.fac[p];
   }
   const double r0 = block_sum((double)redo), r1 = block_sum((double)pend),
-               r2 = block_sum((double)hard);
+               r2 = block_sum((double)hard), r3 = block_sum((double)esc);
   if (threadIdx.x == 0) {
     f.count[1] = (int32_t)r0;
     f.count[2] = (int32_t)r1;
     f.count[3] += (int32_t)r2;
+    f.count[4] = (int32_t)r3;
   }
 }
 
@@ -691,7 +700,7 @@ void b_gmres_cycle(BatchPlan &B) {
 }
 
 void b_solve_end(BatchPlan &B) {
-  hipLaunchKernelGGL(kb_solve_end, dim3(1), dim3(256), 0, B.stream, B.P, B.f);
+  hipLaunchKernelGGL(kb_solve_end, dim3(1), dim3(256), 0, B.stream, B.P, B.d_pd, B.f);
   MAFEM_HIP(hipGetLastError());
 }
 

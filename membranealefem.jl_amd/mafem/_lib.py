@@ -16,6 +16,7 @@ OK, NOT_CONVERGED, EINVAL, EHIP, ENOMEM, EUNSUPPORTED = 0, 1, -1, -2, -3, -4
 i64p = ctypes.POINTER(ctypes.c_int64)
 i32p = ctypes.POINTER(ctypes.c_int32)
 f64p = ctypes.POINTER(ctypes.c_double)
+u8p = ctypes.POINTER(ctypes.c_uint8)
 
 
 class MeshDesc(ctypes.Structure):
@@ -86,6 +87,7 @@ class BatchStats(ctypes.Structure):
         ("gmres_cycles", ctypes.c_int32), ("hard_failures", ctypes.c_int32),
         ("patch_iters", ctypes.c_int64), ("patch_factors", ctypes.c_int64),
         ("gmres_iters", ctypes.c_int64), ("elements", ctypes.c_int64),
+        ("fp64_patches", ctypes.c_int32), ("reserved", ctypes.c_int32),
     ]
 
 
@@ -113,7 +115,7 @@ SIGNATURES = {
                                           ctypes.POINTER(BatchOpts)]),
     "mafem_batch_destroy": (None, [ctypes.c_void_p]),
     "mafem_batch_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
-    "mafem_batch_time_step": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p, ctypes.c_int32,
+    "mafem_batch_time_step": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p, u8p, ctypes.c_int32,
                                              ctypes.c_int32, i32p, f64p, i32p]),
     "mafem_batch_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(BatchStats)]),
 }

@@ -196,15 +196,21 @@ class Batch:
         self.h = h
         self.n = n
 
-    def time_step(self, times, dts, predict=True, max_iter=14):
-        """One backward-Euler step of every member; returns (iters, eps
-        histories, status) with status 0 = converged, 1 = not converged."""
+    def time_step(self, times, dts, predict=True, max_iter=14, active=None):
+        """One backward-Euler step of the selected members (`active`: one flag
+        per member, None = all); returns (iters, eps histories, status) with
+        status 0 = converged, 1 = not converged, 2 = not selected."""
         t = np.ascontiguousarray(np.broadcast_to(np.asarray(times, dtype=np.float64), (self.n,)))
         d = np.ascontiguousarray(np.broadcast_to(np.asarray(dts, dtype=np.float64), (self.n,)))
         it = np.zeros(self.n, dtype=np.int32)
         st = np.zeros(self.n, dtype=np.int32)
         eh = np.zeros((self.n, max_iter))
+        act = None
+        if active is not None:
+            act = np.ascontiguousarray(np.asarray(active, dtype=np.uint8))
+            assert act.shape == (self.n,)
         rc = self.lib.mafem_batch_time_step(self.h, L.ptr(t, L.f64p), L.ptr(d, L.f64p),
+                                            None if act is None else L.ptr(act, L.u8p),
                                             1 if predict else 0, int(max_iter),
                                             L.ptr(it, L.i32p), L.ptr(eh, L.f64p),
                                             L.ptr(st, L.i32p))

@@ -76,6 +76,6 @@ def test_batch_create_rejects_bad_arguments():
     assert lib.mafem_batch_create(ctypes.byref(h), 0, None, None) == _lib.EINVAL
     assert not h.value
     assert b"member" in lib.mafem_batch_last_error(None)
-    assert lib.mafem_batch_time_step(None, None, None, 0, 14, None, None, None) == _lib.EINVAL
+    assert lib.mafem_batch_time_step(None, None, None, None, 0, 14, None, None, None) == _lib.EINVAL
     st = _lib.BatchStats()
     assert lib.mafem_batch_get_stats(None, ctypes.byref(st)) == _lib.EINVAL

@@ -114,3 +114,38 @@ def test_alone_and_in_mix_are_identical():
         assert np.array_equal(one[0][0], mix[k][0]) and np.array_equal(one[0][1], mix[k][1]), k
     for k in range(len(PATCHES)):
         assert np.array_equal(mix[k][1], mix_ng[k][1]), k
+
+
+def test_fp32_failure_moves_patch_to_fp64():
+    """A large centre-refined LAG patch whose tangent is too ill-conditioned
+    for an FP32 factorization: the batch moves it to FP64 storage and its
+    state still matches the FP64 direct path."""
+    from mafem.patches import Patch, PatchSpec
+    print("escalation: LAG 83x83 kb=0.53 zv=8.5 with two small patches", flush=True)
+    specs = [PatchSpec(mafem.LAG, 83, 0.5286395248414045, 8.539281469289673, 0.8995302812009891,
+                       478865490),
+             PatchSpec(mafem.ALEVB, 12, 1.0, 1.0, 0.5, 7), PatchSpec(mafem.LAG, 10, 2.0, 0.5, 0.4, 8)]
+    batch_pts = [Patch(s) for s in specs]
+    b = mafem.Batch([pt.dev for pt in batch_pts])
+    for k in range(2):
+        for pt in batch_pts:
+            pt.set_pull(pt.spec.speed_at(k))
+        it, eps, st = b.time_step((k + 1) * DT, DT)
+        assert (st == 0).all(), (k, st, eps)
+    stats = b.stats()
+    assert stats["fp64_patches"] >= 1
+    got = [pt.dev.get_state() for pt in batch_pts]
+    b.close()
+    for pt, (xb, cb) in zip(batch_pts, got):
+        pt.close()
+    for s, (xb, cb) in zip(specs, got):
+        ref = Patch(s)
+        ref.dev.set_solver("direct", precision=64, rtol=1e-13)
+        for k in range(2):
+            ref.set_pull(s.speed_at(k))
+            ref.dev.predict(DT)
+            rc, _ = ref.dev.time_step((k + 1) * DT, DT)
+            assert rc == 0
+        x, c = ref.dev.get_state()
+        ref.close()
+        assert rel(cb, c) <= RTOL_STATE and rel(xb, x) <= RTOL_STATE
