@@ -1,6 +1,7 @@
 // C ABI of libmafem (declared in include/mafem.h).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -49,7 +50,8 @@ void free_all(Ctx &c) {
                   c.d_rsrc, c.d_r,      c.d_x,       c.d_w,       c.d_t,      c.d_b,
                   c.d_red,  c.d_csc_src, c.d_csc_val, c.d_eq,     c.d_band,   c.d_linv,
                   c.d_uinv, c.d_piv,  c.d_V,       c.d_Z,       c.d_h,      c.d_coef,
-                  c.d_mdot_part, c.d_t2, c.d_presc_idx, c.d_presc_val};
+                  c.d_mdot_part, c.d_t2, c.d_presc_idx, c.d_presc_val, c.d_s1,
+                  c.d_sweep_err};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c.h_red) (void)hipHostFree(c.h_red);
@@ -322,6 +324,28 @@ void build(Ctx &c, const mafem_mesh_desc *m, const mafem_params *p) {
   c.d_eq = dupload(c, eq_ext);
   c.opts = mafem_solver_opts{MAFEM_SOLVER_DIRECT, 64, 30, 200, 20, 3, 1e-13};
   c.d_piv = dalloc<int32_t>(c, c.nt * TILE);
+  // sweeps over several workgroups once the chain of tile rows is long
+  c.sweep_groups = (int)std::min<int64_t>(16, std::max<int64_t>(1, c.nt / 32));
+  if (const char *e = std::getenv("MAFEM_SWEEP_GROUPS")) {
+    const int g = std::atoi(e);
+    if (g >= 1 && g <= 64) c.sweep_groups = g;
+  }
+  c.d_s1 = dalloc<double>(c, nv);
+  c.d_sweep_err = dalloc<int>(c, 1);
+  MAFEM_HIP(hipMemset(c.d_sweep_err, 0, sizeof(int)));
+}
+
+// A multi-workgroup sweep whose waits ran out leaves an error count (its
+// results are invalid): checked at the end of every solve and time step.
+void check_sweeps(Ctx &c) {
+  if (c.sweep_groups <= 1) return;
+  int e = 0;
+  MAFEM_HIP(hipMemcpyAsync(&e, c.d_sweep_err, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+  MAFEM_HIP(hipStreamSynchronize(c.stream));
+  if (e != 0) {
+    MAFEM_HIP(hipMemsetAsync(c.d_sweep_err, 0, sizeof(int), c.stream));
+    throw ArgError{"a triangular sweep wait ran out (results discarded)", MAFEM_EHIP};
+  }
 }
 
 // CSC export over active DOFs, columns in equation order (the pattern of
@@ -631,6 +655,7 @@ int mafem_solve(mafem_ctx *ctx, const double *rhs, double *du, double *relres) {
     MAFEM_HIP(hipGetLastError());
     MAFEM_HIP(hipMemcpyAsync(du, c.d_t, sizeof(double) * c.nmdf, hipMemcpyDeviceToHost, c.stream));
     MAFEM_HIP(hipStreamSynchronize(c.stream));
+    check_sweeps(c);
     if (relres) *relres = rel;
     c.stats.factor_us = 1e3 * elapsed(c, 2, 3);
     c.stats.solve_us = 1e3 * elapsed(c, 3, 4);
@@ -681,6 +706,7 @@ int mafem_time_step(mafem_ctx *ctx, double time, double dt, int32_t max_iter, in
     }
     MAFEM_HIP(hipEventRecord(c.ev[6], c.stream));
     MAFEM_HIP(hipEventSynchronize(c.ev[6]));
+    check_sweeps(c);
     st.total_us = 1e3 * elapsed(c, 0, 6);
     st.newton_iters = it;
     st.n_band_solve = c.n_band_solve - nbs0;

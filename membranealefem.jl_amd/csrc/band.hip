@@ -405,10 +405,9 @@ __device__ __forceinline__ double tile_row4(const S *M, const double *v) {
 // Sum over the band tiles (I, K), K in [K0, K1), of tile_row4 against the ring;
 // up to 12 (FP64) or 24 (FP32) tile loads per thread are issued before their
 // results are used.
-template <class S>
+template <class S, int B = (sizeof(S) == 4 ? 24 : 10)>  // loads in flight within the VGPR budget
 __device__ __forceinline__ double band_row_sum(const S *band, int tb, int64_t I, int64_t K0,
                                                int64_t K1, const double (*ring)[T]) {
-  constexpr int B = sizeof(S) == 4 ? 24 : 12;  // loads in flight within the VGPR budget
   const int i = threadIdx.x >> 4, q = threadIdx.x & 15;
   double acc = 0.0;
   for (int64_t Kb = K0; Kb < K1; Kb += B) {
@@ -499,6 +498,146 @@ template <class S>
 __global__ __launch_bounds__(SW) void k_fwd_sweep(int64_t nt, int tb, const S *__restrict__ band,
                                                   double *__restrict__ y) {
   fwd_sweep_body<S>(nt, tb, band, y);
+}
+
+// ---------------------------------------------------- multi-workgroup sweeps
+//
+// The same recurrences split over W workgroups: workgroup w solves tile rows
+// w, w + W, ... (descending for the backward sweep).  Each solved row is
+// published with relaxed agent-scope atomic stores into an output vector
+// pre-filled with a sentinel, and consumers re-load a row until none of its
+// 64 values is the sentinel: the value is its own ready flag, so no fences
+// are needed.  A row's older tiles are loaded and summed while the row the
+// chain waits for (the neighbour) is still being produced; only the last
+// tile waits for it.  Every wait is bounded: on expiry the kernel counts an
+// error and every workgroup leaves.
+
+constexpr unsigned long long SENTINEL = 0x7FF4DEADBEEF1234ULL;  // a signalling-NaN pattern
+constexpr long SPIN_LIMIT = 20000000L;
+
+__global__ __launch_bounds__(256) void k_fill_sentinel(int64_t n, unsigned long long *__restrict__ y) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t < n) y[t] = SENTINEL;
+}
+
+// rows K of [k0, k1] into the LDS ring, one wave per row
+__device__ __forceinline__ void stage_rows(const unsigned long long *__restrict__ y, int64_t k0,
+                                           int64_t k1, double (*ring)[T], int *bad) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int64_t K = k0 + wave; K <= k1; K += SW / 64) {
+    long spins = 0;
+    unsigned long long v;
+    for (;;) {
+      v = __hip_atomic_load(&y[K * T + lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (__all(v != SENTINEL)) break;
+      if (*bad) break;
+      __builtin_amdgcn_s_sleep(1);
+      if (++spins > SPIN_LIMIT) {
+        *bad = 1;
+        break;
+      }
+    }
+    ring[K % RING][lane] = __longlong_as_double((long long)v);
+  }
+}
+
+template <class S>
+__device__ __forceinline__ double tile_row_fma(const typename Vec4<S>::type &m, const double *x) {
+  return fma((double)m.x, x[0], fma((double)m.y, x[1], fma((double)m.z, x[2], (double)m.w * x[3])));
+}
+
+// y_I = c_I - Σ_{K<I} L~_IK y_K
+template <class S>
+__global__ __launch_bounds__(SW) void k_fwd_sweep_mw(int64_t nt, int tb, const S *__restrict__ band,
+                                                     const double *__restrict__ c,
+                                                     unsigned long long *__restrict__ y,
+                                                     int *__restrict__ err) {
+  __shared__ double ring[RING][T];
+  __shared__ int bad;
+  const int W = gridDim.x;
+  const int i = threadIdx.x >> 4, q = threadIdx.x & 15;
+  if (threadIdx.x == 0) bad = 0;
+  __syncthreads();
+  int64_t have = -1;  // rows 0..have are in the ring
+  for (int64_t I = blockIdx.x; I < nt; I += W) {
+    const int64_t K0 = I - tb < 0 ? 0 : I - tb;
+    typename Vec4<S>::type ml{};
+    if (I >= 1)
+      ml = *reinterpret_cast<const typename Vec4<S>::type *>(tile_ptr(band, tb, I, I - 1) + i * T +
+                                                             4 * q);
+    double acc = 0.0;
+    if (I - 1 > K0) {
+      stage_rows(y, have + 1 > K0 ? have + 1 : K0, I - 2, ring, &bad);
+      if (I - 2 > have) have = I - 2;
+      __syncthreads();
+      if (bad) break;
+      acc = band_row_sum<S, sizeof(S) == 4 ? 12 : 8>(band, tb, I, K0, I - 1, ring);
+    }
+    if (I >= 1) {
+      stage_rows(y, have + 1 > I - 1 ? have + 1 : I - 1, I - 1, ring, &bad);
+      if (I - 1 > have) have = I - 1;
+      __syncthreads();
+      if (bad) break;
+      acc += tile_row_fma<S>(ml, ring[(I - 1) % RING] + 4 * q);
+    }
+    const double s = row16_sum(acc);
+    if (q == 0) {
+      const double u = c[I * T + i] - s;
+      __hip_atomic_store(&y[I * T + i], (unsigned long long)__double_as_longlong(u),
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+  }
+  if (bad && threadIdx.x == 0) atomicAdd(err, 1);
+}
+
+// x_I = d_I - Σ_{J>I} U~_IJ x_J
+template <class S>
+__global__ __launch_bounds__(SW) void k_bwd_sweep_mw(int64_t nt, int tb, const S *__restrict__ band,
+                                                     const double *__restrict__ d,
+                                                     unsigned long long *__restrict__ x,
+                                                     int *__restrict__ err) {
+  __shared__ double ring[RING][T];
+  __shared__ int bad;
+  const int W = gridDim.x;
+  const int i = threadIdx.x >> 4, q = threadIdx.x & 15;
+  if (threadIdx.x == 0) bad = 0;
+  __syncthreads();
+  int64_t have = nt;  // rows have..nt-1 are in the ring
+  for (int64_t I = nt - 1 - blockIdx.x; I >= 0; I -= W) {
+    const int64_t J1 = I + tb > nt - 1 ? nt - 1 : I + tb;
+    typename Vec4<S>::type ml{};
+    if (I + 1 <= nt - 1)
+      ml = *reinterpret_cast<const typename Vec4<S>::type *>(tile_ptr(band, tb, I, I + 1) + i * T +
+                                                             4 * q);
+    double acc = 0.0;
+    if (I + 1 < J1) {
+      // rows J1 down to I + 2, staged from the highest not yet in the ring
+      const int64_t hi = have - 1 < J1 ? have - 1 : J1;
+      stage_rows(x, I + 2, hi, ring, &bad);
+      if (I + 2 < have) have = I + 2;
+      __syncthreads();
+      if (bad) break;
+      acc = band_row_sum<S, sizeof(S) == 4 ? 16 : 8>(band, tb, I, I + 2, J1 + 1, ring);
+    }
+    if (I + 1 <= nt - 1) {
+      if (I + 1 < have) {
+        stage_rows(x, I + 1, I + 1, ring, &bad);
+        have = I + 1;
+      }
+      __syncthreads();
+      if (bad) break;
+      acc += tile_row_fma<S>(ml, ring[(I + 1) % RING] + 4 * q);
+    }
+    const double s = row16_sum(acc);
+    if (q == 0) {
+      const double u = d[I * T + i] - s;
+      __hip_atomic_store(&x[I * T + i], (unsigned long long)__double_as_longlong(u),
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+  }
+  if (bad && threadIdx.x == 0) atomicAdd(err, 1);
 }
 
 template <class S>
@@ -705,6 +844,26 @@ static void factor_t(Ctx &c, S *band, S *li, S *ui) {
 template <class S>
 static void solve_t(Ctx &c, const S *band, const S *li, const S *ui, const double *d_rhs,
                     double *d_x) {
+  if (c.sweep_groups > 1) {
+    // c = L_II^-1 P b -> t2; y (sentinel-filled s1) = forward sweep; d = U_II^-1 y -> t2;
+    // x (sentinel-filled) = backward sweep
+    const int64_t nv = c.nt * T;
+    const unsigned nb = (unsigned)((nv + 255) / 256);
+    hipLaunchKernelGGL(k_diag_apply<S>, dim3((unsigned)c.nt), dim3(256), 0, c.stream, li, c.d_piv,
+                       d_rhs, c.d_t2);
+    hipLaunchKernelGGL(k_fill_sentinel, dim3(nb), dim3(256), 0, c.stream, nv,
+                       (unsigned long long *)c.d_s1);
+    hipLaunchKernelGGL(k_fwd_sweep_mw<S>, dim3(c.sweep_groups), dim3(SW), 0, c.stream, c.nt, c.tb,
+                       band, c.d_t2, (unsigned long long *)c.d_s1, c.d_sweep_err);
+    hipLaunchKernelGGL(k_diag_apply<S>, dim3((unsigned)c.nt), dim3(256), 0, c.stream, ui,
+                       (const int32_t *)nullptr, c.d_s1, c.d_t2);
+    hipLaunchKernelGGL(k_fill_sentinel, dim3(nb), dim3(256), 0, c.stream, nv,
+                       (unsigned long long *)d_x);
+    hipLaunchKernelGGL(k_bwd_sweep_mw<S>, dim3(c.sweep_groups), dim3(SW), 0, c.stream, c.nt, c.tb,
+                       band, c.d_t2, (unsigned long long *)d_x, c.d_sweep_err);
+    MAFEM_HIP(hipGetLastError());
+    return;
+  }
   // y = L^-1 P b in place in d_x, then x = U^-1 y
   hipLaunchKernelGGL(k_diag_apply<S>, dim3((unsigned)c.nt), dim3(256), 0, c.stream, li, c.d_piv,
                      d_rhs, c.d_t2);

@@ -122,6 +122,11 @@ struct Ctx {
   mafem_stats stats{};
   int32_t n_band_solve = 0;     // running count of triangular-solve sweeps
 
+  // triangular sweeps split over sweep_groups workgroups (1: one workgroup)
+  int sweep_groups = 1;
+  double *d_s1 = nullptr;        // sweep output (sentinel-filled before each sweep)
+  int *d_sweep_err = nullptr;    // count of sweep waits that ran out (must stay 0)
+
   // staging of mafem_set_prescribed (grown on demand, kept)
   int64_t *d_presc_idx = nullptr;
   double *d_presc_val = nullptr;

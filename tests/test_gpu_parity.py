@@ -239,3 +239,24 @@ def test_solver_options_validation():
         dev.set_solver("krylov", precision=16)
     with pytest.raises(mafem._lib.LibError):
         dev.set_solver("krylov", restart=0)
+
+
+@pytest.mark.parametrize("groups", [2, 5, 16])
+def test_multi_workgroup_sweeps_match_single(groups, monkeypatch):
+    """The triangular sweeps split over several workgroups (rows published
+    through sentinel-filled outputs) give the single-workgroup solution up to
+    summation order."""
+    print(f"sweep groups {groups}", flush=True)
+    p, mesh, xms, cps, _ = setup(mafem.ALEVB, 16)
+    rhs = np.random.default_rng(3).normal(size=mesh.nmdf)
+    sols = []
+    for g in (1, groups):
+        monkeypatch.setenv("MAFEM_SWEEP_GROUPS", str(g))
+        dev = mafem.DeviceMesh(mesh, p)
+        dev.set_state(xms, cps)
+        dev.assemble(2e-3, 1e-3, want_K=False)
+        x, rel = dev.solve(rhs)
+        assert rel <= 1e-12
+        sols.append(x)
+        dev.close()
+    assert np.linalg.norm(sols[1] - sols[0]) / np.linalg.norm(sols[0]) <= 1e-9
