@@ -19,6 +19,7 @@
 #ifndef MAFEM_H
 #define MAFEM_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -166,6 +167,10 @@ int mafem_get_stats(const mafem_ctx *ctx, mafem_stats *out);
  * 3 refinement sweeps, rtol 1e-13. */
 int mafem_set_solver(mafem_ctx *ctx, const mafem_solver_opts *opts);
 int mafem_get_solver(const mafem_ctx *ctx, mafem_solver_opts *opts);
+
+/* Free and total memory of the current device, in bytes (no counterpart in
+ * the reference: used to admit work by HBM budget). */
+int mafem_device_memory(size_t *free_bytes, size_t *total_bytes);
 
 /* ------------------------------------------------------------------ batches
  *

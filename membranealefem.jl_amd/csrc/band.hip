@@ -879,6 +879,8 @@ static void ensure_band(Ctx &c, int prec) {
   const int64_t ntile = c.nt * (2 * c.tb + 1) * (int64_t)T * T, ninv = c.nt * (int64_t)T * T;
   const size_t es = (prec == 32) ? sizeof(float) : sizeof(double);
   if (c.band_prec == prec && c.d_band) return;
+  c.batch_factor_prec = 0;
+  c.have_factor = false;
   if (c.d_band) {
     MAFEM_HIP(hipStreamSynchronize(c.stream));
     (void)hipFree(c.d_band);
@@ -904,6 +906,7 @@ void band_alloc(Ctx &c, int prec) { ensure_band(c, prec); }
 
 void band_fill(Ctx &c, int prec) {
   ensure_band(c, prec);
+  c.batch_factor_prec = 0;  // the band no longer holds a batch's factorization
   if (prec == 32)
     fill_t<float>(c, (float *)c.d_band);
   else

@@ -389,6 +389,11 @@ static thread_local std::string g_batch_err;
 
 extern "C" {
 
+int mafem_device_memory(size_t *free_bytes, size_t *total_bytes) {
+  if (!free_bytes || !total_bytes) return MAFEM_EINVAL;
+  return hipMemGetInfo(free_bytes, total_bytes) == hipSuccess ? MAFEM_OK : MAFEM_EHIP;
+}
+
 int mafem_batch_create(mafem_batch **out, int32_t n, mafem_ctx *const *members,
                        const mafem_batch_opts *opts) {
   if (!out) return MAFEM_EINVAL;
@@ -425,9 +430,21 @@ int mafem_batch_create(mafem_batch **out, int32_t n, mafem_ctx *const *members,
       MAFEM_HIP(hipStreamCreateWithFlags(&b.B.stream, hipStreamNonBlocking));
       for (auto &e : b.ev) MAFEM_HIP(hipEventCreate(&e));
       MAFEM_HIP(hipHostMalloc((void **)&b.h_count, 32, hipHostMallocDefault));
-      b.prec.assign(n, o.factor_precision);
+      // a member keeps the precision a previous batch chose for it and, when
+      // still valid, the factorization that batch left in its band
+      b.prec.resize(n);
+      for (int32_t i = 0; i < n; ++i) {
+        const Ctx &c = members[i]->c;
+        b.prec[i] = c.batch_prec == 64 ? 64 : o.factor_precision;
+      }
       alloc_flags(b);
       build_plan(b);
+      std::vector<uint8_t> fac(n);
+      for (int32_t i = 0; i < n; ++i) {
+        const Ctx &c = members[i]->c;
+        fac[i] = (c.batch_factor_prec == b.prec[i] && c.band_prec == b.prec[i]) ? 0 : 1;
+      }
+      MAFEM_HIP(hipMemcpy(b.B.f.fac, fac.data(), n, hipMemcpyHostToDevice));
     } catch (const HipError &h) {
       b.err = std::string("HIP error ") + hipGetErrorString(h.code) + " in " + h.what;
       rc = MAFEM_EHIP;
@@ -589,6 +606,16 @@ int mafem_batch_time_step(mafem_batch *bp, const double *time, const double *dt,
       if (eps_hist)
         for (int k = 0; k < max_iter; ++k)
           eps_hist[(size_t)p * max_iter + k] = k < h_it[p] ? h_hist[(size_t)p * max_iter + k] : 0.0;
+    }
+    // what each member's band holds now, for a later batch
+    std::vector<uint8_t> h_fac(P);
+    MAFEM_HIP(hipMemcpy(h_fac.data(), f.fac, P, hipMemcpyDeviceToHost));
+    for (int p = 0; p < P; ++p) {
+      Ctx &c = b.members[p]->c;
+      c.batch_prec = b.prec[p];
+      c.batch_factor_prec = (h_fac[p] == 0 && h_nf[p] > 0) || c.batch_factor_prec == b.prec[p]
+                                ? (h_fac[p] == 0 ? b.prec[p] : 0)
+                                : 0;
     }
     b.stats = st;
     if (any_fail) {

@@ -122,6 +122,12 @@ struct Ctx {
   mafem_stats stats{};
   int32_t n_band_solve = 0;     // running count of triangular-solve sweeps
 
+  // factorization left in the band by a batch (kept when the context joins
+  // another batch): its precision, 0 = none; batch_prec = precision a batch
+  // last chose for this context (an FP64 fallback stays FP64)
+  int batch_factor_prec = 0;
+  int batch_prec = 0;
+
   // triangular sweeps split over sweep_groups workgroups (1: one workgroup)
   int sweep_groups = 1;
   double *d_s1 = nullptr;        // sweep output (sentinel-filled before each sweep)

@@ -110,6 +110,8 @@ SIGNATURES = {
     "mafem_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Stats)]),
     "mafem_set_solver": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SolverOpts)]),
     "mafem_get_solver": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SolverOpts)]),
+    "mafem_device_memory": (ctypes.c_int, [ctypes.POINTER(ctypes.c_size_t),
+                                           ctypes.POINTER(ctypes.c_size_t)]),
     "mafem_batch_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int32,
                                           ctypes.POINTER(ctypes.c_void_p),
                                           ctypes.POINTER(BatchOpts)]),
