@@ -241,6 +241,150 @@ def bench_batched(args):
     print(json.dumps(res), flush=True)
 
 
+def bench_serving(args):
+    """Serving form: a Poisson stream of requests (each a patch advanced
+    `--req-steps` steps; n from {32, 48, 64, 96, 128}, formulation uniform
+    over LAG, EUL, ALEVB, parameters as in the batched form) served by
+    continuous batching under an HBM budget.  Reports per-step latency
+    percentiles over every member-step, request latency, aggregate Newton
+    throughput, and the plain FP64 direct path's Newton throughput on a
+    sample of the same requests (a bounded number of steps each)."""
+    import mafem
+    from mafem.patches import Patch, serving_specs
+    from mafem.serving import Server, poisson_requests
+
+    specs = serving_specs(args.requests, seed=0xC5)
+    reqs = poisson_requests(specs, rate=args.rate, steps=args.req_steps, seed=0xC5)
+    srv = Server(precision=args.precision if args.precision else 32, max_members=args.max_members)
+    out = srv.run(reqs, log=lambda m: print("serve: " + m, file=sys.stderr, flush=True))
+    done = [r for r in out if r.finished is not None and r.failed < 0]
+    failed = [{"rid": r.rid, "n": r.spec.n, "motion": int(r.spec.motion), "step": r.failed}
+              for r in out if r.failed >= 0]
+    lat = np.array(srv.step_latency)
+    req_lat = np.array([r.finished - r.arrival for r in done])
+    queue_wait = np.array([r.started - r.arrival for r in done])
+    value = srv.newton_iters / srv.wall
+
+    # plain FP64 direct path on a sample of the requests, a few steps each
+    plain_iters, plain_time = 0, 0.0
+    idx = np.linspace(0, len(specs) - 1, min(args.sample, len(specs))).astype(int)
+    for i in idx:
+        pt = Patch(specs[i])
+        pt.dev.set_solver("direct", precision=64, rtol=1e-13)
+        for k in range(args.plain_steps):
+            pt.set_pull(specs[i].speed_at(k))
+            ts = time.perf_counter()
+            pt.dev.predict(1e-3)
+            rc, eps = pt.dev.time_step((k + 1) * 1e-3, 1e-3)
+            plain_time += time.perf_counter() - ts
+            plain_iters += len(eps)
+            if rc != 0:
+                break
+        pt.close()
+    plain_rate = plain_iters / plain_time
+    res = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "Newton steps/s",
+        "n_gpus": 1,
+        "steps": int(sum(len(r.iters) for r in out)),
+        "warmup": 0,
+        "ms_per_step": 1e3 * float(np.mean(lat)) if len(lat) else None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (FP32 band-LU preconditioner)",
+        "data": "synthetic",
+        "config": {"workload": f"serving: {len(reqs)} requests, Poisson arrivals at {args.rate}/s, "
+                               f"{args.req_steps} steps each, n in {{32,48,64,96,128}}, motion "
+                               f"U{{LAG, EUL, ALEVB}}, kb and zv log-uniform in [0.1, 10], pull "
+                               f"ramped to U[0.2, 3] over 200 steps, seed 0xC5, HBM-budget "
+                               f"admission, continuous batching",
+                   "requests": len(reqs), "served": len(done)},
+        "step_latency_ms_p50": 1e3 * float(np.percentile(lat, 50)),
+        "step_latency_ms_p99": 1e3 * float(np.percentile(lat, 99)),
+        "request_latency_s_p50": float(np.percentile(req_lat, 50)) if len(done) else None,
+        "request_latency_s_p99": float(np.percentile(req_lat, 99)) if len(done) else None,
+        "queue_wait_s_p50": float(np.percentile(queue_wait, 50)) if len(done) else None,
+        "wall_s": srv.wall, "batch_rebuilds": srv.rebuilds, "rebuild_s": srv.rebuild_time,
+        "max_members": int(max(srv.step_members)) if srv.step_members else 0,
+        "hbm_budget_gb": srv.budget / 1e9, "peak_admitted_gb": srv.peak_bytes / 1e9,
+        "failed_requests": failed,
+        "plain_fp64": {"newton_per_s": plain_rate, "sample": int(len(idx)),
+                       "steps_each": args.plain_steps,
+                       "path": "single-context direct FP64, one request at a time"},
+        "speedup_vs_plain_fp64": value / plain_rate,
+    }
+    print(json.dumps(res), flush=True)
+
+
+def bench_precision(args):
+    """Precision form: one large patch (n = --n, default 256) advanced with the
+    mixed-precision solver (FP64 FGMRES, FP32 band-LU preconditioner kept
+    across iterations; FP64 assembly, residual, update and ε test) and, from
+    the same initial state, with the all-FP64 direct path; reports the
+    per-step relative state error between the two and both throughputs."""
+    import mafem
+    n = args.n if args.n != 64 else 256
+    dt = 1e-3
+    out = {}
+    for mode in ("mixed", "fp64"):
+        p, mesh, xms, cps = build_case(n, args.motion, sigma=0.0)
+        dev = mafem.DeviceMesh(mesh, p)
+        if mode == "mixed":
+            dev.set_solver("krylov", precision=32, restart=30, max_iters=300, refactor_iters=40,
+                           rtol=1e-12)
+        else:
+            dev.set_solver("direct", precision=64, rtol=1e-13)
+        dev.set_state(xms, cps)
+        pres = [(node, mesh.dofs[u]) for (u, node, _v) in mesh.inh_dir_bcs]
+        states, iters, times, stats = [], [], [], []
+        for k in range(args.steps):
+            dev.set_prescribed([a for a, _ in pres], [b for _, b in pres], [pull_speed(k)] * len(pres))
+            ts = time.perf_counter()
+            dev.predict(dt)
+            rc, eps = dev.time_step((k + 1) * dt, dt)
+            times.append(time.perf_counter() - ts)
+            if rc != 0:
+                raise RuntimeError(f"{mode}: Newton did not converge at step {k}: {eps}")
+            iters.append(len(eps))
+            stats.append(dev.stats())
+            states.append(dev.get_state())
+            print(f"bench: {mode} step {k}: {times[-1]:.2f} s, {len(eps)} Newton iterations, "
+                  f"factorizations {stats[-1]['n_factor']}, gmres {stats[-1]['gmres_iters']}",
+                  file=sys.stderr, flush=True)
+        out[mode] = (states, iters, times, stats, mesh)
+        dev.close()
+    sm, im, tm, stm, mesh = out["mixed"]
+    sf, iF, tf, stf, _ = out["fp64"]
+    errs = [max(np.linalg.norm(a[0] - b[0]) / np.linalg.norm(b[0]),
+                np.linalg.norm(a[1] - b[1]) / np.linalg.norm(b[1])) for a, b in zip(sm, sf)]
+    res = {
+        "metric": METRIC,
+        "value": sum(im) / sum(tm),
+        "unit": "Newton steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": 0,
+        "ms_per_step": 1e3 * sum(tm) / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (FP32 band-LU preconditioner in FP64 FGMRES)",
+        "data": "synthetic",
+        "config": {"workload": f"F_PULL {args.motion} {n}x{n} patch (fixed pull location), flat "
+                               f"initial state, pull ramped to 2.0 over 200 steps, dt=1e-3",
+                   "nmdf": mesh.nmdf, "numel": mesh.numel},
+        "newton_iters_mixed": im, "newton_iters_fp64": iF,
+        "per_step_rel_err_vs_fp64": errs,
+        "fp64_direct_newton_per_s": sum(iF) / sum(tf),
+        "step_s_mixed": tm, "step_s_fp64": tf,
+        "gmres_iters_mixed": [s["gmres_iters"] for s in stm],
+        "factorizations_mixed": [s["n_factor"] for s in stm],
+    }
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -252,7 +396,13 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--solver", default="direct", choices=["direct", "krylov"])
     ap.add_argument("--precision", type=int, default=None, choices=[64, 32])
-    ap.add_argument("--workload", default="single", choices=["single", "batched"])
+    ap.add_argument("--workload", default="single",
+                    choices=["single", "batched", "serving", "precision"])
+    ap.add_argument("--requests", type=int, default=48)
+    ap.add_argument("--rate", type=float, default=200.0)
+    ap.add_argument("--req-steps", type=int, default=50)
+    ap.add_argument("--max-members", type=int, default=64)
+    ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--patches", type=int, default=512)
     ap.add_argument("--nmin", type=int, default=16)
     ap.add_argument("--nmax", type=int, default=96)
@@ -263,6 +413,12 @@ def main():
               "(independent patches shard across GPUs with no communication; see DESIGN.md)",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "serving":
+        bench_serving(args)
+        return
+    if args.workload == "precision":
+        bench_precision(args)
+        return
     if args.workload == "batched":
         if args.precision is None:
             args.precision = 32

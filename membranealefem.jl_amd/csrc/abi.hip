@@ -50,7 +50,7 @@ void free_all(Ctx &c) {
                   c.d_rsrc, c.d_r,      c.d_x,       c.d_w,       c.d_t,      c.d_b,
                   c.d_red,  c.d_csc_src, c.d_csc_val, c.d_eq,     c.d_band,   c.d_linv,
                   c.d_uinv, c.d_piv,  c.d_V,       c.d_Z,       c.d_h,      c.d_coef,
-                  c.d_mdot_part, c.d_t2, c.d_presc_idx, c.d_presc_val, c.d_s1,
+                  c.d_mdot_part, c.d_t2, c.d_presc_idx, c.d_presc_val, c.d_s1, c.d_s2,
                   c.d_sweep_err};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -331,6 +331,7 @@ void build(Ctx &c, const mafem_mesh_desc *m, const mafem_params *p) {
     if (g >= 1 && g <= 64) c.sweep_groups = g;
   }
   c.d_s1 = dalloc<double>(c, nv);
+  c.d_s2 = dalloc<double>(c, nv);
   c.d_sweep_err = dalloc<int>(c, 1);
   MAFEM_HIP(hipMemset(c.d_sweep_err, 0, sizeof(int)));
 }
@@ -446,7 +447,15 @@ double linear_solve(Ctx &c, const double *d_b, double *d_x) {
   if (o.mode == MAFEM_SOLVER_DIRECT) {
     refactor(c);
     MAFEM_HIP(hipEventRecord(c.ev[3], c.stream));
-    return solve_refined(c, d_b, d_x, o.refine_sweeps, o.rtol);
+    const double rel = solve_refined(c, d_b, d_x, o.refine_sweeps, o.rtol);
+    if (rel <= 1e3 * o.rtol) return rel;
+    // refinement stalls when the tile-local pivoting grew the factors: finish
+    // with FGMRES preconditioned by the same fresh factorization
+    ensure_krylov(c);
+    int its = 0;
+    const double rel2 = fgmres(c, d_b, d_x, &its);
+    c.stats.last_relres = rel2;
+    return rel2;
   }
   bool fresh = false;
   if (!c.have_factor || c.band_prec != o.factor_precision) {

@@ -435,10 +435,18 @@ template <class S>
 __device__ __forceinline__ void diag_apply_body(int64_t bid, const S *__restrict__ inv,
                                                     const int32_t *__restrict__ piv,
                                                     const double *__restrict__ v,
-                                                    double *__restrict__ out) {
+                                                    double *__restrict__ out,
+                                                    bool atomic_in = false) {
   __shared__ double sv[T];
   const int64_t I = bid;
-  if (threadIdx.x < T) sv[threadIdx.x] = v[I * T + (piv ? piv[I * T + threadIdx.x] : threadIdx.x)];
+  if (threadIdx.x < T) {
+    const int64_t k = I * T + (piv ? piv[I * T + threadIdx.x] : threadIdx.x);
+    // the output of a multi-workgroup sweep is only ever accessed atomically
+    sv[threadIdx.x] = atomic_in ? __longlong_as_double((long long)__hip_atomic_load(
+                                      reinterpret_cast<const unsigned long long *>(v) + k,
+                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                                : v[k];
+  }
   __syncthreads();
   const int i = threadIdx.x / 4, q = threadIdx.x % 4;
   const S *m = inv + I * (int64_t)T * T + i * T + q * 16;
@@ -456,6 +464,13 @@ __global__ __launch_bounds__(256) void k_diag_apply(const S *__restrict__ inv,
                                                     const double *__restrict__ v,
                                                     double *__restrict__ out) {
   diag_apply_body<S>((int64_t)blockIdx.x, inv, piv, v, out);
+}
+
+template <class S>
+__global__ __launch_bounds__(256) void k_diag_apply_atomic(const S *__restrict__ inv,
+                                                           const double *__restrict__ v,
+                                                           double *__restrict__ out) {
+  diag_apply_body<S>((int64_t)blockIdx.x, inv, (const int32_t *)nullptr, v, out, true);
 }
 
 // y_I = c_I - Σ_{K<I} L~_IK y_K  (c precomputed into y)
@@ -515,9 +530,21 @@ __global__ __launch_bounds__(SW) void k_fwd_sweep(int64_t nt, int tb, const S *_
 constexpr unsigned long long SENTINEL = 0x7FF4DEADBEEF1234ULL;  // a signalling-NaN pattern
 constexpr long SPIN_LIMIT = 20000000L;
 
+// The sweep outputs are written and read only with agent-scope atomics
+// (never through a cached plain access), so no stale cache line can stand in
+// for the sentinel or for a published row.
 __global__ __launch_bounds__(256) void k_fill_sentinel(int64_t n, unsigned long long *__restrict__ y) {
   const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (t < n) y[t] = SENTINEL;
+  if (t < n) __hip_atomic_store(&y[t], SENTINEL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// dst = src for a sweep output, read atomically, written as plain doubles
+__global__ __launch_bounds__(256) void k_collect(int64_t n, const unsigned long long *__restrict__ src,
+                                                 double *__restrict__ dst) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t < n)
+    dst[t] = __longlong_as_double((long long)__hip_atomic_load(&src[t], __ATOMIC_RELAXED,
+                                                               __HIP_MEMORY_SCOPE_AGENT));
 }
 
 // rows K of [k0, k1] into the LDS ring, one wave per row
@@ -548,18 +575,18 @@ __device__ __forceinline__ double tile_row_fma(const typename Vec4<S>::type &m, 
 
 // y_I = c_I - Σ_{K<I} L~_IK y_K
 template <class S>
-__global__ __launch_bounds__(SW) void k_fwd_sweep_mw(int64_t nt, int tb, const S *__restrict__ band,
-                                                     const double *__restrict__ c,
-                                                     unsigned long long *__restrict__ y,
-                                                     int *__restrict__ err) {
+__device__ __forceinline__ void fwd_sweep_mw_body(int w, int W, int64_t nt, int tb,
+                                                  const S *__restrict__ band,
+                                                  const double *__restrict__ c,
+                                                  unsigned long long *__restrict__ y,
+                                                  int *__restrict__ err) {
   __shared__ double ring[RING][T];
   __shared__ int bad;
-  const int W = gridDim.x;
   const int i = threadIdx.x >> 4, q = threadIdx.x & 15;
   if (threadIdx.x == 0) bad = 0;
   __syncthreads();
   int64_t have = -1;  // rows 0..have are in the ring
-  for (int64_t I = blockIdx.x; I < nt; I += W) {
+  for (int64_t I = w; I < nt; I += W) {
     const int64_t K0 = I - tb < 0 ? 0 : I - tb;
     typename Vec4<S>::type ml{};
     if (I >= 1)
@@ -593,18 +620,18 @@ __global__ __launch_bounds__(SW) void k_fwd_sweep_mw(int64_t nt, int tb, const S
 
 // x_I = d_I - Σ_{J>I} U~_IJ x_J
 template <class S>
-__global__ __launch_bounds__(SW) void k_bwd_sweep_mw(int64_t nt, int tb, const S *__restrict__ band,
-                                                     const double *__restrict__ d,
-                                                     unsigned long long *__restrict__ x,
-                                                     int *__restrict__ err) {
+__device__ __forceinline__ void bwd_sweep_mw_body(int w, int W, int64_t nt, int tb,
+                                                  const S *__restrict__ band,
+                                                  const double *__restrict__ d,
+                                                  unsigned long long *__restrict__ x,
+                                                  int *__restrict__ err) {
   __shared__ double ring[RING][T];
   __shared__ int bad;
-  const int W = gridDim.x;
   const int i = threadIdx.x >> 4, q = threadIdx.x & 15;
   if (threadIdx.x == 0) bad = 0;
   __syncthreads();
   int64_t have = nt;  // rows have..nt-1 are in the ring
-  for (int64_t I = nt - 1 - blockIdx.x; I >= 0; I -= W) {
+  for (int64_t I = nt - 1 - w; I >= 0; I -= W) {
     const int64_t J1 = I + tb > nt - 1 ? nt - 1 : I + tb;
     typename Vec4<S>::type ml{};
     if (I + 1 <= nt - 1)
@@ -638,6 +665,22 @@ __global__ __launch_bounds__(SW) void k_bwd_sweep_mw(int64_t nt, int tb, const S
     __syncthreads();
   }
   if (bad && threadIdx.x == 0) atomicAdd(err, 1);
+}
+
+template <class S>
+__global__ __launch_bounds__(SW) void k_fwd_sweep_mw(int64_t nt, int tb, const S *__restrict__ band,
+                                                     const double *__restrict__ c,
+                                                     unsigned long long *__restrict__ y,
+                                                     int *__restrict__ err) {
+  fwd_sweep_mw_body<S>(blockIdx.x, gridDim.x, nt, tb, band, c, y, err);
+}
+
+template <class S>
+__global__ __launch_bounds__(SW) void k_bwd_sweep_mw(int64_t nt, int tb, const S *__restrict__ band,
+                                                     const double *__restrict__ d,
+                                                     unsigned long long *__restrict__ x,
+                                                     int *__restrict__ err) {
+  bwd_sweep_mw_body<S>(blockIdx.x, gridDim.x, nt, tb, band, d, x, err);
 }
 
 template <class S>
@@ -782,6 +825,9 @@ __global__ __launch_bounds__(256) void kb_diag_apply(Seg sg, const PatchDev *__r
   const PatchDev &d = pd[p];
   if (which == 0)
     diag_apply_body<S>(blockIdx.x - sg.blk0[p], (const S *)d.linv, d.piv, d.w, d.t2);
+  else if (d.sweep_w > 1)  // forward result in s1 (atomic); the backward sweep reads t2
+    diag_apply_body<S>(blockIdx.x - sg.blk0[p], (const S *)d.uinv, (const int32_t *)nullptr, d.s1,
+                       d.t2, true);
   else
     diag_apply_body<S>(blockIdx.x - sg.blk0[p], (const S *)d.uinv, (const int32_t *)nullptr, d.t2,
                        d.t);
@@ -791,7 +837,7 @@ template <class S>
 __global__ __launch_bounds__(SW) void kb_fwd_sweep(const PatchDev *__restrict__ pd,
                                                    const uint8_t *__restrict__ mask) {
   const int p = blockIdx.x;
-  if (!mask[p] || pd[p].prec != prec_of<S>()) return;
+  if (!mask[p] || pd[p].prec != prec_of<S>() || pd[p].sweep_w > 1) return;
   fwd_sweep_body<S>(pd[p].nt, pd[p].tb, (const S *)pd[p].band, pd[p].t2);
 }
 
@@ -799,8 +845,51 @@ template <class S>
 __global__ __launch_bounds__(SW) void kb_bwd_sweep(const PatchDev *__restrict__ pd,
                                                    const uint8_t *__restrict__ mask) {
   const int p = blockIdx.x;
-  if (!mask[p] || pd[p].prec != prec_of<S>()) return;
+  if (!mask[p] || pd[p].prec != prec_of<S>() || pd[p].sweep_w > 1) return;
   bwd_sweep_body<S>(pd[p].nt, pd[p].tb, (const S *)pd[p].band, pd[p].t);
+}
+
+// Patches with sweep_w > 1 sweep over sweep_w workgroups (segment `sg`);
+// their outputs are sentinel-filled first.  which = 0: forward sweep of t2
+// into s1; which = 1: backward sweep of t2 into s2 (then collected into t).
+__global__ __launch_bounds__(256) void kb_fill_sentinel(Seg sg, const PatchDev *__restrict__ pd,
+                                                        const uint8_t *__restrict__ mask, int which,
+                                                        int prec) {
+  const int p = sg.map[blockIdx.x];
+  if (!mask[p] || pd[p].sweep_w <= 1 || pd[p].prec != prec) return;
+  const PatchDev &d = pd[p];
+  const int64_t t = (int64_t)(blockIdx.x - sg.blk0[p]) * 256 + threadIdx.x;
+  if (t < d.nt * T)
+    __hip_atomic_store(reinterpret_cast<unsigned long long *>(which == 0 ? d.s1 : d.s2) + t,
+                       SENTINEL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// t = backward-sweep output (atomic reads) for multi-workgroup patches
+__global__ __launch_bounds__(256) void kb_collect(Seg sg, const PatchDev *__restrict__ pd,
+                                                  const uint8_t *__restrict__ mask, int prec) {
+  const int p = sg.map[blockIdx.x];
+  if (!mask[p] || pd[p].sweep_w <= 1 || pd[p].prec != prec) return;
+  const PatchDev &d = pd[p];
+  const int64_t t = (int64_t)(blockIdx.x - sg.blk0[p]) * 256 + threadIdx.x;
+  if (t < d.nt * T)
+    d.t[t] = __longlong_as_double((long long)__hip_atomic_load(
+        reinterpret_cast<const unsigned long long *>(d.s2) + t, __ATOMIC_RELAXED,
+        __HIP_MEMORY_SCOPE_AGENT));
+}
+
+template <class S>
+__global__ __launch_bounds__(SW) void kb_sweep_mw(Seg sg, const PatchDev *__restrict__ pd,
+                                                  const uint8_t *__restrict__ mask, int which) {
+  const int p = sg.map[blockIdx.x];
+  if (!mask[p] || pd[p].prec != prec_of<S>()) return;
+  const PatchDev &d = pd[p];
+  const int w = (int)(blockIdx.x - sg.blk0[p]);
+  if (which == 0)
+    fwd_sweep_mw_body<S>(w, d.sweep_w, d.nt, d.tb, (const S *)d.band, d.t2,
+                         reinterpret_cast<unsigned long long *>(d.s1), d.err);
+  else
+    bwd_sweep_mw_body<S>(w, d.sweep_w, d.nt, d.tb, (const S *)d.band, d.t2,
+                         reinterpret_cast<unsigned long long *>(d.s2), d.err);
 }
 
 // ------------------------------------------------------------------ host side
@@ -846,7 +935,7 @@ static void solve_t(Ctx &c, const S *band, const S *li, const S *ui, const doubl
                     double *d_x) {
   if (c.sweep_groups > 1) {
     // c = L_II^-1 P b -> t2; y (sentinel-filled s1) = forward sweep; d = U_II^-1 y -> t2;
-    // x (sentinel-filled) = backward sweep
+    // x (sentinel-filled s2) = backward sweep, collected into d_x
     const int64_t nv = c.nt * T;
     const unsigned nb = (unsigned)((nv + 255) / 256);
     hipLaunchKernelGGL(k_diag_apply<S>, dim3((unsigned)c.nt), dim3(256), 0, c.stream, li, c.d_piv,
@@ -855,12 +944,14 @@ static void solve_t(Ctx &c, const S *band, const S *li, const S *ui, const doubl
                        (unsigned long long *)c.d_s1);
     hipLaunchKernelGGL(k_fwd_sweep_mw<S>, dim3(c.sweep_groups), dim3(SW), 0, c.stream, c.nt, c.tb,
                        band, c.d_t2, (unsigned long long *)c.d_s1, c.d_sweep_err);
-    hipLaunchKernelGGL(k_diag_apply<S>, dim3((unsigned)c.nt), dim3(256), 0, c.stream, ui,
-                       (const int32_t *)nullptr, c.d_s1, c.d_t2);
+    hipLaunchKernelGGL(k_diag_apply_atomic<S>, dim3((unsigned)c.nt), dim3(256), 0, c.stream, ui,
+                       c.d_s1, c.d_t2);
     hipLaunchKernelGGL(k_fill_sentinel, dim3(nb), dim3(256), 0, c.stream, nv,
-                       (unsigned long long *)d_x);
+                       (unsigned long long *)c.d_s2);
     hipLaunchKernelGGL(k_bwd_sweep_mw<S>, dim3(c.sweep_groups), dim3(SW), 0, c.stream, c.nt, c.tb,
-                       band, c.d_t2, (unsigned long long *)d_x, c.d_sweep_err);
+                       band, c.d_t2, (unsigned long long *)c.d_s2, c.d_sweep_err);
+    hipLaunchKernelGGL(k_collect, dim3(nb), dim3(256), 0, c.stream, nv,
+                       (const unsigned long long *)c.d_s2, d_x);
     MAFEM_HIP(hipGetLastError());
     return;
   }
@@ -976,12 +1067,25 @@ void b_factor(BatchPlan &B) {
 
 template <class S>
 static void b_band_solve_t(BatchPlan &B, const uint8_t *mask) {
+  const bool mw = B.mw.nblk > 0;
   hipLaunchKernelGGL(kb_diag_apply<S>, dim3(B.tiles.nblk), dim3(256), 0, B.stream, B.tiles, B.d_pd,
                      mask, 0);
   hipLaunchKernelGGL(kb_fwd_sweep<S>, dim3(B.P), dim3(SW), 0, B.stream, B.d_pd, mask);
+  if (mw) {
+    hipLaunchKernelGGL(kb_fill_sentinel, dim3(B.vec.nblk), dim3(256), 0, B.stream, B.vec, B.d_pd,
+                       mask, 0, prec_of<S>());
+    hipLaunchKernelGGL(kb_sweep_mw<S>, dim3(B.mw.nblk), dim3(SW), 0, B.stream, B.mw, B.d_pd, mask, 0);
+  }
   hipLaunchKernelGGL(kb_diag_apply<S>, dim3(B.tiles.nblk), dim3(256), 0, B.stream, B.tiles, B.d_pd,
                      mask, 1);
   hipLaunchKernelGGL(kb_bwd_sweep<S>, dim3(B.P), dim3(SW), 0, B.stream, B.d_pd, mask);
+  if (mw) {
+    hipLaunchKernelGGL(kb_fill_sentinel, dim3(B.vec.nblk), dim3(256), 0, B.stream, B.vec, B.d_pd,
+                       mask, 1, prec_of<S>());
+    hipLaunchKernelGGL(kb_sweep_mw<S>, dim3(B.mw.nblk), dim3(SW), 0, B.stream, B.mw, B.d_pd, mask, 1);
+    hipLaunchKernelGGL(kb_collect, dim3(B.vec.nblk), dim3(256), 0, B.stream, B.vec, B.d_pd, mask,
+                       prec_of<S>());
+  }
   MAFEM_HIP(hipGetLastError());
 }
 

@@ -9,6 +9,7 @@
 // graphs stay valid for every iteration and step.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <numeric>
@@ -120,7 +121,28 @@ PatchDev describe(const Ctx &c) {
   d.r = c.d_r; d.b = c.d_b; d.x = c.d_x; d.w = c.d_w; d.t = c.d_t; d.t2 = c.d_t2;
   d.band = c.d_band; d.linv = c.d_linv; d.uinv = c.d_uinv; d.piv = c.d_piv;
   d.prec = c.band_prec;
+  d.sweep_w = 1;
+  d.s1 = c.d_s1;
+  d.s2 = c.d_s2;
+  d.err = c.d_sweep_err;
   return d;
+}
+
+// Workgroups of each patch's sweeps: the longest chains get several (up to
+// 16, one per 64 tile rows) while the total stays within what the device
+// keeps resident at once (one 1024-thread sweep workgroup per CU), so every
+// workgroup a sweep waits on is running.
+void assign_sweep_groups(std::vector<PatchDev> &pd, int budget) {
+  std::vector<int> order(pd.size());
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return pd[x].nt > pd[y].nt; });
+  for (int p : order) {
+    const int w = (int)std::min<int64_t>(16, pd[p].nt / 64);
+    if (w > 1 && w <= budget) {
+      pd[p].sweep_w = w;
+      budget -= w;
+    }
+  }
 }
 
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
@@ -177,6 +199,7 @@ void build_plan(mafem_batch &b) {
   B.P = P;
   B.m = b.opts.restart;
   B.refactor_iters = b.opts.refactor_iters;
+  B.escalate_iters = b.opts.escalate_iters;
   B.rtol = b.opts.rtol;
   b.h_pd.resize(P);
   B.has32 = B.has64 = false;
@@ -187,6 +210,11 @@ void build_plan(mafem_batch &b) {
     b.h_pd[p].V = balloc<double>(b, (B.m + 1) * c.nt * (int64_t)TILE);
     (b.prec[p] == 32 ? B.has32 : B.has64) = true;
   }
+  int dev_cus = 0;
+  MAFEM_HIP(hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, b.dev));
+  int mw_budget = (3 * dev_cus) / 4;
+  if (const char *e = std::getenv("MAFEM_BATCH_SWEEP_BUDGET")) mw_budget = std::atoi(e);
+  assign_sweep_groups(b.h_pd, mw_budget);
   B.d_pd = bupload(b, b.h_pd);
 
   // segment maps
@@ -223,6 +251,9 @@ void build_plan(mafem_batch &b) {
   B.zero = make_seg(b, bz);
   B.tiles = make_seg(b, bt);
   B.fold = make_seg(b, bf);
+  std::vector<int64_t> bw(P);
+  for (int p = 0; p < P; ++p) bw[p] = b.h_pd[p].sweep_w > 1 ? b.h_pd[p].sweep_w : 0;
+  B.mw = make_seg(b, bw);
 
   // factorization order: by precision, then classes of equal tb, each by
   // decreasing nt
@@ -402,10 +433,11 @@ int mafem_batch_create(mafem_batch **out, int32_t n, mafem_ctx *const *members,
     g_batch_err = "a batch needs at least one member";
     return MAFEM_EINVAL;
   }
-  mafem_batch_opts o{32, 20, 3, 12, 1, 0, 1e-12};
+  mafem_batch_opts o{32, 20, 1, 6, 1, 12, 1e-10};
   if (opts) o = *opts;
   if ((o.factor_precision != 32 && o.factor_precision != 64) || o.restart < 1 ||
-      o.restart > 64 || o.max_cycles < 1 || o.refactor_iters < 0 || !(o.rtol > 0.0)) {
+      o.restart > 64 || o.max_cycles < 1 || o.refactor_iters < 0 || o.escalate_iters < 1 ||
+      !(o.rtol > 0.0)) {
     g_batch_err = "invalid batch options";
     return MAFEM_EINVAL;
   }
@@ -560,6 +592,11 @@ int mafem_batch_time_step(mafem_batch *bp, const double *time, const double *dt,
       read_counts(b);
       t_asm += ms_between(b.ev[1], b.ev[2]);
       t_upd += ms_between(b.ev[3], b.ev[4]);
+      if (b.h_count[4] > 0) {
+        escalate(b);
+        b_count(B);
+        read_counts(b);
+      }
       pend = b.h_count[2];
       if (b.h_count[0] == 0) {
         ++it;
@@ -606,6 +643,16 @@ int mafem_batch_time_step(mafem_batch *bp, const double *time, const double *dt,
       if (eps_hist)
         for (int k = 0; k < max_iter; ++k)
           eps_hist[(size_t)p * max_iter + k] = k < h_it[p] ? h_hist[(size_t)p * max_iter + k] : 0.0;
+    }
+    // multi-workgroup sweeps whose waits ran out invalidate the step
+    for (int p = 0; p < P; ++p) {
+      if (b.h_pd[p].sweep_w <= 1) continue;
+      int e = 0;
+      MAFEM_HIP(hipMemcpy(&e, b.h_pd[p].err, sizeof(int), hipMemcpyDeviceToHost));
+      if (e != 0) {
+        MAFEM_HIP(hipMemset(b.h_pd[p].err, 0, sizeof(int)));
+        throw BatchError{"a triangular sweep wait ran out (results discarded)", MAFEM_EHIP};
+      }
     }
     // what each member's band holds now, for a later batch
     std::vector<uint8_t> h_fac(P);

@@ -130,7 +130,7 @@ struct Ctx {
 
   // triangular sweeps split over sweep_groups workgroups (1: one workgroup)
   int sweep_groups = 1;
-  double *d_s1 = nullptr;        // sweep output (sentinel-filled before each sweep)
+  double *d_s1 = nullptr, *d_s2 = nullptr;  // sweep outputs (sentinel-filled, atomic access only)
   int *d_sweep_err = nullptr;    // count of sweep waits that ran out (must stay 0)
 
   // staging of mafem_set_prescribed (grown on demand, kept)
@@ -168,7 +168,9 @@ struct PatchDev {
   int32_t *piv;
   double *V;         // GMRES basis, m + 1 vectors of length nt TILE (batch-owned)
   int32_t prec;      // band storage precision of this patch: 32 or 64
-  int32_t pad1;
+  int32_t sweep_w;   // workgroups of its triangular sweeps
+  double *s1, *s2;   // forward/backward outputs of multi-workgroup sweeps (atomic access only)
+  int *err;          // count of sweep waits that ran out
 };
 
 // Block -> patch map of one batched launch.
@@ -225,8 +227,9 @@ struct BatchPlan {
   bool has32 = false, has64 = false;  // precision groups present
   int max_iter = 14;
   int m = 20;                // GMRES basis size
-  int refactor_iters = 12;
-  double rtol = 1e-12;
+  int refactor_iters = 6;
+  int escalate_iters = 12;
+  double rtol = 1e-10;
   hipStream_t stream = nullptr;
   PatchDev *d_pd = nullptr;
   int32_t *d_order = nullptr;
@@ -234,6 +237,7 @@ struct BatchPlan {
   std::vector<int> motions;         // motions present
   std::vector<Seg> area, edge;      // per motion present
   Seg bsr, vec, node, zero, tiles, fold;
+  Seg mw;                           // blocks of the multi-workgroup sweeps
   std::vector<std::vector<StepMap>> steps32, steps64;  // factorization steps per precision group
   int64_t max_nt = 0;
 };

@@ -576,12 +576,26 @@ __global__ __launch_bounds__(256) void kb_mark_factored(int P, BatchFlags f) {
 
 // refactor before the next solve when this one needed more than
 // refactor_iters Arnoldi steps on a factorization that was not fresh
-__global__ __launch_bounds__(256) void kb_refactor_policy(int P, BatchFlags f, int refactor_iters) {
+// After a converged solve: a stale factorization that needed more than
+// refactor_iters Arnoldi steps is refactored before the next solve; a fresh
+// FP32 one that needed more than escalate_iters moves the patch to FP64
+// (count[4] = such moves).
+__global__ __launch_bounds__(256) void kb_refactor_policy(int P, const PatchDev *__restrict__ pd,
+                                                          BatchFlags f, int refactor_iters,
+                                                          int escalate_iters) {
+  int esc = 0;
   for (int p = threadIdx.x; p < P; p += 256)
     if (f.on[p]) {
       if (!f.fresh[p] && f.kits[p] > refactor_iters) f.fac[p] = 1;
+      if (f.fresh[p] && pd[p].prec == 32 && f.kits[p] > escalate_iters) {
+        f.esc[p] = 1;
+        f.fac[p] = 1;
+        ++esc;
+      }
       f.fresh[p] = 0;
     }
+  const double r = block_sum((double)esc);
+  if (threadIdx.x == 0) f.count[4] = (int32_t)r;
 }
 
 // cps += Δ, xms += Δt Δ[:, motion] (FiniteElement.jl:41-46), one node per thread
@@ -710,8 +724,8 @@ void b_mark_factored(BatchPlan &B) {
 }
 
 void b_update(BatchPlan &B) {
-  hipLaunchKernelGGL(kb_refactor_policy, dim3(1), dim3(256), 0, B.stream, B.P, B.f,
-                     B.refactor_iters);
+  hipLaunchKernelGGL(kb_refactor_policy, dim3(1), dim3(256), 0, B.stream, B.P, B.d_pd, B.f,
+                     B.refactor_iters, B.escalate_iters);
   hipLaunchKernelGGL(kb_update_state, dim3(B.node.nblk), dim3(256), 0, B.stream, B.node, B.d_pd,
                      B.f.on, B.f.dt);
   hipLaunchKernelGGL(kb_sumsq_x, dim3(B.vec.nblk), dim3(256), 0, B.stream, B.vec, B.d_pd, B.f.on,

@@ -73,7 +73,7 @@ class BatchOpts(ctypes.Structure):
     _fields_ = [
         ("factor_precision", ctypes.c_int32), ("restart", ctypes.c_int32),
         ("max_cycles", ctypes.c_int32), ("refactor_iters", ctypes.c_int32),
-        ("use_graphs", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("use_graphs", ctypes.c_int32), ("escalate_iters", ctypes.c_int32),
         ("rtol", ctypes.c_double),
     ]
 

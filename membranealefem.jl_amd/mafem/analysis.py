@@ -181,14 +181,14 @@ class Batch:
     borrowed: keep them alive while the batch exists.
     """
 
-    def __init__(self, members, precision=32, restart=20, max_cycles=3, refactor_iters=12,
-                 use_graphs=True, rtol=1e-12):
+    def __init__(self, members, precision=32, restart=20, max_cycles=1, refactor_iters=6,
+                 use_graphs=True, rtol=1e-10, escalate_iters=12):
         self.lib = L.load()
         self.members = list(members)
         n = len(self.members)
         arr = (ctypes.c_void_p * n)(*[m.h.value for m in self.members])
         o = L.BatchOpts(int(precision), int(restart), int(max_cycles), int(refactor_iters),
-                        1 if use_graphs else 0, 0, float(rtol))
+                        1 if use_graphs else 0, int(escalate_iters), float(rtol))
         h = ctypes.c_void_p()
         rc = self.lib.mafem_batch_create(ctypes.byref(h), n, arr, ctypes.byref(o))
         if rc != L.OK:

@@ -72,6 +72,7 @@ class Server:
         self.step_members = []     # members of every batch step
         self.newton_iters = 0
         self.rebuilds = 0
+        self.rebuild_time = 0.0
         self.peak_bytes = 0
 
     def _bytes(self, r):
@@ -140,9 +141,11 @@ class Server:
                 if changed:
                     if batch is not None:
                         batch.close()
+                    tb0 = time.perf_counter()
                     batch = Batch([m[1].dev for m in members], precision=self.precision,
                                   use_graphs=self.use_graphs)
                     self.rebuilds += 1
+                    self.rebuild_time += time.perf_counter() - tb0
                     changed = False
                 times = np.array([(m[2] + 1) * self.dt for m in members])
                 for r, pt, k in members:
@@ -152,6 +155,10 @@ class Server:
                 el = time.perf_counter() - ts
                 self.step_latency += [el] * len(members)
                 self.step_members.append(len(members))
+                if log is not None:
+                    log(f"t={time.perf_counter() - t0:.2f}s step of {len(members)} members "
+                        f"{el:.3f}s, {int(it.sum())} Newton iterations, pending {len(pending)}, "
+                        f"building {building}")
                 keep = []
                 for j, m in enumerate(members):
                     r, pt, k = m
@@ -172,9 +179,6 @@ class Server:
                         changed = True
                     else:
                         keep.append(m)
-                if log is not None and changed:
-                    log(f"t={time.perf_counter() - t0:.2f}s members {len(keep)} "
-                        f"pending {len(pending)} building {building}")
                 members = keep
         finally:
             if batch is not None:

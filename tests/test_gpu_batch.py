@@ -149,3 +149,36 @@ def test_fp32_failure_moves_patch_to_fp64():
         x, c = ref.dev.get_state()
         ref.close()
         assert rel(cb, c) <= RTOL_STATE and rel(xb, x) <= RTOL_STATE
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_batch_multi_workgroup_sweeps_match_single_patch(graphs):
+    """Long chains inside a batch sweep over several workgroups; a 48x48 ALEVB
+    patch (one of them) next to small ones still matches the FP64 direct path."""
+    from mafem.patches import Patch, PatchSpec
+    print("batch with a multi-workgroup sweep member", flush=True)
+    specs = [PatchSpec(mafem.ALEVB, 48, 1.0, 1.0, 0.5, 11), PatchSpec(mafem.LAG, 8, 1.0, 1.0, 0.5, 12),
+             PatchSpec(mafem.EUL, 48, 2.0, 0.5, 0.4, 13), PatchSpec(mafem.LAG, 48, 0.5, 2.0, 0.6, 14),
+             PatchSpec(mafem.ALEVB, 64, 1.0, 1.0, 0.3, 15)]
+    pts = [Patch(s) for s in specs]
+    b = mafem.Batch([pt.dev for pt in pts], use_graphs=graphs)
+    for k in range(2):
+        for pt in pts:
+            pt.set_pull(pt.spec.speed_at(k))
+        it, eps, st = b.time_step((k + 1) * DT, DT)
+        print(f"  step {k}: iters {list(it)} status {list(st)} stats {b.stats()}", flush=True)
+        assert (st == 0).all(), (k, st, eps)
+    got = [pt.dev.get_state() for pt in pts]
+    b.close()
+    for pt in pts:
+        pt.close()
+    for s, (xb, cb) in zip(specs, got):
+        ref = Patch(s)
+        ref.dev.set_solver("direct", precision=64, rtol=1e-13)
+        for k in range(2):
+            ref.set_pull(s.speed_at(k))
+            ref.dev.predict(DT)
+            assert ref.dev.time_step((k + 1) * DT, DT)[0] == 0
+        x, c = ref.dev.get_state()
+        ref.close()
+        assert rel(cb, c) <= RTOL_STATE and rel(xb, x) <= RTOL_STATE

@@ -260,3 +260,27 @@ def test_multi_workgroup_sweeps_match_single(groups, monkeypatch):
         sols.append(x)
         dev.close()
     assert np.linalg.norm(sols[1] - sols[0]) / np.linalg.norm(sols[0]) <= 1e-9
+
+
+@pytest.mark.parametrize("motion,n,groups", [(mafem.ALEVB, 48, 8), (mafem.LAG, 64, 16),
+                                              (mafem.EUL, 40, 6), (mafem.LAG, 48, 2)])
+def test_multi_workgroup_sweeps_larger_patches(motion, n, groups, monkeypatch):
+    """Multi-workgroup sweeps on wider bands: the direct solve (band LU + FP64
+    refinement) of the Newton right-hand side agrees with the
+    single-workgroup sweep's."""
+    print(f"sweep groups {groups} motion={motion.name} n={n}", flush=True)
+    p = mafem.Params(motion=motion, num1el=n, num2el=n, length=1.0, enr=1e-10, output=False)
+    mesh = mafem.make_mesh(p, pull_speed=0.3)
+    xms, cps = mafem.prepare_input(p, mesh, pull_speed=0.3)
+    sols = []
+    for g in (1, groups):
+        monkeypatch.setenv("MAFEM_SWEEP_GROUPS", str(g))
+        dev = mafem.DeviceMesh(mesh, p)
+        dev.set_state(xms, cps)
+        r, _ = dev.assemble(1e-3, 1e-3, want_K=False)
+        x, rel = dev.solve(-r)
+        print(f"  groups {g}: relres {rel:.2e}", flush=True)
+        assert rel <= 1e-10
+        sols.append(x)
+        dev.close()
+    assert np.linalg.norm(sols[1] - sols[0]) / np.linalg.norm(sols[0]) <= 1e-8
