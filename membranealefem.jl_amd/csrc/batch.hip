@@ -330,10 +330,13 @@ bool plan_stale(const mafem_batch &b) {
   return false;
 }
 
+// Recording uses the relaxed capture mode: contexts may be created on other
+// host threads (synchronous copies on the null stream) while a batch records
+// its launch sequence on its own non-blocking stream.
 template <class F>
 hipGraphExec_t record(mafem_batch &b, F &&issue) {
   hipGraph_t g = nullptr;
-  MAFEM_HIP(hipStreamBeginCapture(b.B.stream, hipStreamCaptureModeThreadLocal));
+  MAFEM_HIP(hipStreamBeginCapture(b.B.stream, hipStreamCaptureModeRelaxed));
   try {
     issue();
   } catch (...) {
