@@ -14,6 +14,11 @@ using namespace mafem;
 
 static thread_local std::string g_create_err;
 
+std::mutex &mafem::capture_mutex() {
+  static std::mutex m;
+  return m;
+}
+
 namespace {
 
 struct ArgError {
@@ -506,6 +511,7 @@ int mafem_create(mafem_ctx **out, const mafem_mesh_desc *mesh, const mafem_param
     return MAFEM_ENOMEM;
   }
   int rc = MAFEM_OK;
+  std::lock_guard<std::mutex> no_capture(capture_mutex());
   try {
     int ndev = 0;
     MAFEM_HIP(hipGetDeviceCount(&ndev));
@@ -543,6 +549,7 @@ const char *mafem_last_error(const mafem_ctx *ctx) {
 
 int mafem_set_state(mafem_ctx *ctx, const double *xms, const double *cps) {
   ABI_GUARD(ctx, {
+    std::lock_guard<std::mutex> no_capture(capture_mutex());
     if (!xms || !cps) return fail(c, "null state", MAFEM_EINVAL);
     MAFEM_HIP(hipSetDevice(c.dev));
     MAFEM_HIP(hipMemcpyAsync(c.d_xms, xms, sizeof(double) * c.numnp * 3, hipMemcpyHostToDevice, c.stream));
@@ -554,6 +561,7 @@ int mafem_set_state(mafem_ctx *ctx, const double *xms, const double *cps) {
 
 int mafem_get_state(mafem_ctx *ctx, double *xms, double *cps) {
   ABI_GUARD(ctx, {
+    std::lock_guard<std::mutex> no_capture(capture_mutex());
     MAFEM_HIP(hipSetDevice(c.dev));
     if (xms)
       MAFEM_HIP(hipMemcpyAsync(xms, c.d_xms, sizeof(double) * c.numnp * 3, hipMemcpyDeviceToHost, c.stream));
@@ -567,6 +575,7 @@ int mafem_get_state(mafem_ctx *ctx, double *xms, double *cps) {
 int mafem_set_prescribed(mafem_ctx *ctx, int64_t n, const int64_t *node, const int32_t *dof,
                          const double *val) {
   ABI_GUARD(ctx, {
+    std::lock_guard<std::mutex> no_capture(capture_mutex());
     if (n < 0 || (n > 0 && (!node || !dof || !val))) return fail(c, "bad prescribed list", MAFEM_EINVAL);
     MAFEM_HIP(hipSetDevice(c.dev));
     std::vector<int64_t> idx(n);

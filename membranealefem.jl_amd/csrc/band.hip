@@ -107,16 +107,23 @@ __device__ __forceinline__ float lane_bcast(float x, int l) {
 
 // LU with partial pivoting of the 64x64 diagonal tile in one wavefront: lane
 // i holds row i in registers (staged through LDS so the global accesses are
-// coalesced).  Rows are not moved during the elimination: at step j the
-// candidate row with the largest |a_ij| (pivot_key) is the pivot, its row is
-// broadcast element by element with readlane, and every other candidate row
-// is eliminated (non-candidates use a zero multiplier, which leaves them
-// unchanged).  The row chosen at step j is stored as row j of the factored
-// tile (P A = L U) and piv[K][j] records its original index.
+// coalesced).  Rows are not moved: the candidate row with the largest |a_ij|
+// (pivot_key) is the pivot of step j, and the row chosen at step j is stored
+// as row j of the factored tile (P A = L U); piv[K][j] records its original
+// index.  Blocked in four panels of 16 columns: the 16 pivot steps of a panel
+// eliminate only inside the panel (the pivot row's entries broadcast with
+// readlane, non-candidates using a zero multiplier), then the panel's pivot
+// rows publish their trailing entries and multipliers to LDS, U12 = L11^-1
+// A12 is solved with one lane per trailing column, and every remaining
+// candidate row takes the rank-16 update A22 -= L21 U12 from broadcast LDS
+// reads.  Same arithmetic as the unblocked elimination, with the per-step
+// chain shortened from 64 - j to at most 15 columns.
 template <class S>
 __device__ __forceinline__ void diag_body(int64_t K, int tb, S *__restrict__ band,
-                                             int32_t *__restrict__ piv) {
+                                          int32_t *__restrict__ piv) {
   __shared__ S st[T][T + 1];
+  __shared__ S u12[16][T];
+  __shared__ S l11[16][16];
   const int lane = threadIdx.x;
   S *A = tile_ptr(band, tb, K, K);
 #pragma unroll 8
@@ -127,15 +134,56 @@ __device__ __forceinline__ void diag_body(int64_t K, int tb, S *__restrict__ ban
   for (int k = 0; k < T; ++k) r[k] = st[lane][k];
   int pos = -1;
 #pragma unroll
-  for (int j = 0; j < T; ++j) {
-    const unsigned key = wave_max_u32(pivot_key(r[j], pos < 0, lane));
-    const int p = 63 - (int)(key & 63u);
-    const S pj = lane_bcast(r[j], p);
-    if (lane == p) pos = j;
-    const S l = (pos < 0) ? r[j] / pj : S(0);
-    if (pos < 0) r[j] = l;
+  for (int pb = 0; pb < 4; ++pb) {
+    const int c0 = pb * 16;
 #pragma unroll
-    for (int k = j + 1; k < T; ++k) r[k] = fma(-l, lane_bcast(r[k], p), r[k]);
+    for (int jj = 0; jj < 16; ++jj) {
+      const int j = c0 + jj;
+      const unsigned key = wave_max_u32(pivot_key(r[j], pos < 0, lane));
+      const int p = 63 - (int)(key & 63u);
+      const S pj = lane_bcast(r[j], p);
+      if (lane == p) pos = j;
+      const S l = (pos < 0) ? r[j] / pj : S(0);
+      if (pos < 0) r[j] = l;
+#pragma unroll
+      for (int k = j + 1; k < c0 + 16; ++k) r[k] = fma(-l, lane_bcast(r[k], p), r[k]);
+    }
+    if (pb == 3) break;
+    const bool mine = pos >= c0 && pos < c0 + 16;  // pivot row of this panel
+    if (mine) {
+      const int i = pos - c0;
+#pragma unroll
+      for (int k = c0 + 16; k < T; ++k) u12[i][k] = r[k];
+#pragma unroll
+      for (int jj = 0; jj < 16; ++jj) l11[i][jj] = (jj < i) ? r[c0 + jj] : S(0);
+    }
+    __syncthreads();
+    if (lane >= c0 + 16) {  // U12 = L11^-1 A12, one trailing column per lane
+      S u[16];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        S acc = u12[i][lane];
+#pragma unroll
+        for (int ii = 0; ii < i; ++ii) acc = fma(-l11[i][ii], u[ii], acc);
+        u[i] = acc;
+      }
+#pragma unroll
+      for (int i = 0; i < 16; ++i) u12[i][lane] = u[i];
+    }
+    __syncthreads();
+    if (pos < 0) {  // A22 -= L21 U12
+#pragma unroll
+      for (int k = c0 + 16; k < T; ++k) {
+        S acc = r[k];
+#pragma unroll
+        for (int jj = 0; jj < 16; ++jj) acc = fma(-r[c0 + jj], u12[jj][k], acc);
+        r[k] = acc;
+      }
+    } else if (mine) {
+#pragma unroll
+      for (int k = c0 + 16; k < T; ++k) r[k] = u12[pos - c0][k];
+    }
+    __syncthreads();
   }
   __syncthreads();
 #pragma unroll

@@ -330,11 +330,12 @@ bool plan_stale(const mafem_batch &b) {
   return false;
 }
 
-// Recording uses the relaxed capture mode: contexts may be created on other
-// host threads (synchronous copies on the null stream) while a batch records
-// its launch sequence on its own non-blocking stream.
+// Recording holds the capture mutex, so no context is being created on
+// another host thread meanwhile (its synchronous null-stream copies would
+// invalidate the capture).
 template <class F>
 hipGraphExec_t record(mafem_batch &b, F &&issue) {
+  std::lock_guard<std::mutex> no_create(capture_mutex());
   hipGraph_t g = nullptr;
   MAFEM_HIP(hipStreamBeginCapture(b.B.stream, hipStreamCaptureModeRelaxed));
   try {

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -254,6 +255,11 @@ void b_update(BatchPlan &B);       // refactor policy, state update, ε, Newton 
 void b_mark_factored(BatchPlan &B);
 void b_count(BatchPlan &B);        // count[0] = on, count[2] = on && fac
 void band_alloc(Ctx &c, int prec); // band storage of the given precision
+
+// Serialises graph recording against context creation on other host threads
+// (creation makes synchronous null-stream copies, which would invalidate a
+// capture in progress).
+std::mutex &capture_mutex();
 
 // kernels / launchers (assemble.hip)
 int launch_assemble(Ctx &c, double time, double dt);
