@@ -121,7 +121,10 @@ def bench_batched(args):
         patches.append(Patch(sp))
         if (k + 1) % 64 == 0:
             print(f"bench: {k + 1}/{len(specs)} patches on the device", file=sys.stderr, flush=True)
-    batch = mafem.Batch([pt.dev for pt in patches], precision=args.precision)
+    if args.refactor_iters is None:
+        args.refactor_iters = 6
+    batch = mafem.Batch([pt.dev for pt in patches], precision=args.precision,
+                        refactor_iters=args.refactor_iters)
     setup_s = time.perf_counter() - t0
     dt = 1e-3
 
@@ -219,6 +222,7 @@ def bench_batched(args):
                                f"to U[0.2, 3] over 200 steps, motion U{{LAG, ALEVB}}, seed 0xC4, "
                                f"flat initial state, dt=1e-3, one batch, HIP graph replay",
                    "patches": len(specs), "elements": int(numel.sum()),
+                   "band_lu_precision": args.precision, "refactor_iters": args.refactor_iters,
                    "time_steps": args.steps, "newton_iters": iters},
         "elem_assemblies_per_s": elems / t_total,
         "time_steps_per_s": args.steps / t_total,
@@ -255,7 +259,8 @@ def bench_serving(args):
 
     specs = serving_specs(args.requests, seed=0xC5)
     reqs = poisson_requests(specs, rate=args.rate, steps=args.req_steps, seed=0xC5)
-    srv = Server(precision=args.precision if args.precision else 32, max_members=args.max_members)
+    srv = Server(precision=args.precision if args.precision else 64, max_members=args.max_members,
+                 refactor_iters=args.refactor_iters if args.refactor_iters is not None else -1)
     out = srv.run(reqs, log=lambda m: print("serve: " + m, file=sys.stderr, flush=True))
     done = [r for r in out if r.finished is not None and r.failed < 0]
     failed = [{"rid": r.rid, "n": r.spec.n, "motion": int(r.spec.motion), "step": r.failed}
@@ -293,14 +298,16 @@ def bench_serving(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 (FP32 band-LU preconditioner)",
+        "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"serving: {len(reqs)} requests, Poisson arrivals at {args.rate}/s, "
                                f"{args.req_steps} steps each, n in {{32,48,64,96,128}}, motion "
                                f"U{{LAG, EUL, ALEVB}}, kb and zv log-uniform in [0.1, 10], pull "
                                f"ramped to U[0.2, 3] over 200 steps, seed 0xC5, HBM-budget "
                                f"admission, continuous batching",
-                   "requests": len(reqs), "served": len(done)},
+                   "requests": len(reqs), "served": len(done),
+                   "band_lu_precision": srv.precision, "refactor_iters": srv.refactor_iters,
+                   "solo_requests": srv.solo_count},
         "step_latency_ms_p50": 1e3 * float(np.percentile(lat, 50)),
         "step_latency_ms_p99": 1e3 * float(np.percentile(lat, 99)),
         "request_latency_s_p50": float(np.percentile(req_lat, 50)) if len(done) else None,
@@ -403,6 +410,9 @@ def main():
     ap.add_argument("--req-steps", type=int, default=50)
     ap.add_argument("--max-members", type=int, default=64)
     ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--refactor-iters", type=int, default=None,
+                    help="batches: refactor after a solve needing more Arnoldi steps (-1: every "
+                         "Newton iteration)")
     ap.add_argument("--patches", type=int, default=512)
     ap.add_argument("--nmin", type=int, default=16)
     ap.add_argument("--nmax", type=int, default=96)

@@ -195,7 +195,8 @@ typedef struct {
   int32_t factor_precision;  /* 32 (default) or 64 */
   int32_t restart;           /* GMRES basis size m (default 20, at most 64) */
   int32_t max_cycles;        /* restart cycles per solve before refactoring (default 1) */
-  int32_t refactor_iters;    /* refactor when a solve needed more Arnoldi steps (default 6) */
+  int32_t refactor_iters;    /* refactor when a solve needed more Arnoldi steps (default 6;
+                                -1: factor afresh for every Newton iteration) */
   int32_t use_graphs;        /* 1 (default): replay recorded HIP graphs */
   int32_t escalate_iters;    /* move a patch to FP64 when a solve on a fresh FP32
                                 factorization needed more Arnoldi steps (default 12) */

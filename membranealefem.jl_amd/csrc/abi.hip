@@ -537,6 +537,7 @@ int mafem_create(mafem_ctx **out, const mafem_mesh_desc *mesh, const mafem_param
 
 void mafem_destroy(mafem_ctx *ctx) {
   if (!ctx) return;
+  std::lock_guard<std::mutex> no_capture(capture_mutex());
   (void)hipSetDevice(ctx->c.dev);
   if (ctx->c.stream) (void)hipStreamSynchronize(ctx->c.stream);
   free_all(ctx->c);

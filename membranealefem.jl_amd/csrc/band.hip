@@ -1018,6 +1018,7 @@ static void ensure_band(Ctx &c, int prec) {
   const int64_t ntile = c.nt * (2 * c.tb + 1) * (int64_t)T * T, ninv = c.nt * (int64_t)T * T;
   const size_t es = (prec == 32) ? sizeof(float) : sizeof(double);
   if (c.band_prec == prec && c.d_band) return;
+  std::lock_guard<std::mutex> no_capture(capture_mutex());  // allocation: not during a capture
   c.batch_factor_prec = 0;
   c.have_factor = false;
   if (c.d_band) {

@@ -440,7 +440,7 @@ int mafem_batch_create(mafem_batch **out, int32_t n, mafem_ctx *const *members,
   mafem_batch_opts o{32, 20, 1, 6, 1, 12, 1e-10};
   if (opts) o = *opts;
   if ((o.factor_precision != 32 && o.factor_precision != 64) || o.restart < 1 ||
-      o.restart > 64 || o.max_cycles < 1 || o.refactor_iters < 0 || o.escalate_iters < 1 ||
+      o.restart > 64 || o.max_cycles < 1 || o.refactor_iters < -1 || o.escalate_iters < 1 ||
       !(o.rtol > 0.0)) {
     g_batch_err = "invalid batch options";
     return MAFEM_EINVAL;

@@ -15,6 +15,7 @@ namespace mafem {
 void ensure_krylov(Ctx &c) {
   const int m = std::max(1, std::min(64, (int)c.opts.restart));
   if (c.d_V && c.kry_m == m) return;
+  std::lock_guard<std::mutex> no_capture(capture_mutex());  // allocation: not during a capture
   if (c.d_V) {
     MAFEM_HIP(hipStreamSynchronize(c.stream));
     (void)hipFree(c.d_V);

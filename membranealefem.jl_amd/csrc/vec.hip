@@ -586,7 +586,8 @@ __global__ __launch_bounds__(256) void kb_refactor_policy(int P, const PatchDev 
   int esc = 0;
   for (int p = threadIdx.x; p < P; p += 256)
     if (f.on[p]) {
-      if (!f.fresh[p] && f.kits[p] > refactor_iters) f.fac[p] = 1;
+      // refactor_iters < 0: a fresh factorization for every Newton iteration
+      if (refactor_iters < 0 || (!f.fresh[p] && f.kits[p] > refactor_iters)) f.fac[p] = 1;
       if (f.fresh[p] && pd[p].prec == 32 && f.kits[p] > escalate_iters) {
         f.esc[p] = 1;
         f.fac[p] = 1;

@@ -42,6 +42,7 @@ class Request:
     eps: list = field(default_factory=list)     # ε history per step
     state: tuple = None        # final (xms, cps)
     failed: int = -1           # step that did not converge, -1 = none
+    error: object = None       # exception raised while serving it
 
 
 def device_memory():
@@ -55,10 +56,16 @@ def device_memory():
 
 
 class Server:
-    """Continuous-batching request loop over one device."""
+    """Continuous-batching request loop over one device.
 
-    def __init__(self, hbm_budget=None, precision=32, dt=1e-3, max_members=64, use_graphs=True,
-                 keep_states=True):
+    Defaults from measurements on the serving mix (n from 32 to 128): batched
+    members factor afresh in FP64 for every Newton iteration (so each GMRES
+    solve takes one or two steps; the tangents of large centre-refined meshes
+    are too ill-conditioned for an FP32 factorization to pay off), and
+    requests of 96x96 elements or more run solo on their own streams."""
+
+    def __init__(self, hbm_budget=None, precision=64, dt=1e-3, max_members=64, use_graphs=True,
+                 keep_states=True, solo_min_elements=96 * 96, max_solo=3, refactor_iters=-1):
         if hbm_budget is None:
             free, _ = device_memory()
             hbm_budget = int(0.8 * free)
@@ -68,16 +75,24 @@ class Server:
         self.max_members = max_members
         self.use_graphs = use_graphs
         self.keep_states = keep_states
+        # Large requests run "solo": each on its own host thread and stream with
+        # the direct FP64 path (fresh factorization every Newton iteration),
+        # overlapping on the device with the batch of the others; the longest
+        # factorization and sweep chains then no longer gate every batch step.
+        self.refactor_iters = refactor_iters
+        self.solo_min_elements = solo_min_elements
+        self.max_solo = max_solo
         self.step_latency = []     # wall time of every batch step (s), one entry per member-step
         self.step_members = []     # members of every batch step
         self.newton_iters = 0
         self.rebuilds = 0
         self.rebuild_time = 0.0
         self.peak_bytes = 0
+        self.solo_count = 0
 
     def _bytes(self, r):
-        # FP64 storage for the band keeps the estimate safe if the member falls back
-        return int(1.15 * device_bytes(r.spec, precision=self.precision))
+        # FP64 band storage in the estimate: a member may fall back to it
+        return int(1.15 * device_bytes(r.spec, precision=64))
 
     def run(self, requests, time_limit=None, log=None):
         """Serve `requests` (sorted by arrival); returns them with results."""
@@ -99,16 +114,50 @@ class Server:
         worker.start()
         used, building = 0, 0
         members = []               # [request, patch, step index]
-        batch, changed = None, False
+        solos = []                 # [thread, request, patch]
+        lock = threading.Lock()    # guards the statistics the solo threads update
         t0 = time.perf_counter()
+
+        def solo(r, pt):
+            try:
+                pt.dev.set_solver("direct", precision=64)
+                for k in range(r.steps):
+                    pt.set_pull(pt.spec.speed_at(k))
+                    ts = time.perf_counter()
+                    pt.dev.predict(self.dt)
+                    rc, eps = pt.dev.time_step((k + 1) * self.dt, self.dt)
+                    el = time.perf_counter() - ts
+                    r.iters.append(len(eps))
+                    r.eps.append(eps)
+                    with lock:
+                        self.step_latency.append(el)
+                        if rc == 0:
+                            self.newton_iters += len(eps)
+                    if rc != 0:
+                        r.failed = k
+                        break
+                if self.keep_states:
+                    r.state = pt.dev.get_state()
+            except Exception as e:  # reported by the serving thread
+                r.error = e
+            r.finished = time.perf_counter() - t0
+        batch, changed = None, False
         try:
-            while pending or building or members:
+            while pending or building or members or solos:
+                # solo requests that finished release their memory
+                for sl in [x for x in solos if not x[0].is_alive()]:
+                    th, r, pt = sl
+                    solos.remove(sl)
+                    if getattr(r, "error", None) is not None:
+                        raise r.error
+                    pt.close()
+                    used -= self._bytes(r)
                 now = time.perf_counter() - t0
                 if time_limit is not None and now > time_limit:
                     break
                 # admission: arrived requests that fit the budget
                 while (pending and pending[0].arrival <= now
-                       and len(members) + building < self.max_members
+                       and len(members) + len(solos) + building < self.max_members
                        and used + self._bytes(pending[0]) <= self.budget):
                     r = pending.popleft()
                     r.admitted = now
@@ -129,21 +178,31 @@ class Server:
                     if isinstance(pt, Exception):
                         raise pt
                     r.started = time.perf_counter() - t0
+                    if (pt.mesh.numel >= self.solo_min_elements
+                            and len(solos) < self.max_solo):
+                        th = threading.Thread(target=solo, args=(r, pt), daemon=True)
+                        solos.append([th, r, pt])
+                        th.start()
+                        self.solo_count += 1
+                        continue
                     members.append([r, pt, 0])
                     changed = True
                 if not members:
-                    if pending and not building:
-                        if pending[0].arrival > now:
-                            time.sleep(min(0.01, pending[0].arrival - now))
-                        elif used + self._bytes(pending[0]) > self.budget:
-                            raise RuntimeError(f"request {pending[0].rid} does not fit the budget")
+                    if building:
+                        continue
+                    if pending and pending[0].arrival > now:
+                        time.sleep(min(0.01, pending[0].arrival - now))
+                    elif pending and used + self._bytes(pending[0]) > self.budget and not solos:
+                        raise RuntimeError(f"request {pending[0].rid} does not fit the budget")
+                    elif solos:
+                        time.sleep(0.005)  # solo requests hold the device memory meanwhile
                     continue
                 if changed:
                     if batch is not None:
                         batch.close()
                     tb0 = time.perf_counter()
                     batch = Batch([m[1].dev for m in members], precision=self.precision,
-                                  use_graphs=self.use_graphs)
+                                  use_graphs=self.use_graphs, refactor_iters=self.refactor_iters)
                     self.rebuilds += 1
                     self.rebuild_time += time.perf_counter() - tb0
                     changed = False
@@ -153,7 +212,8 @@ class Server:
                 ts = time.perf_counter()
                 it, eps, st = batch.time_step(times, self.dt, predict=True)
                 el = time.perf_counter() - ts
-                self.step_latency += [el] * len(members)
+                with lock:
+                    self.step_latency += [el] * len(members)
                 self.step_members.append(len(members))
                 if log is not None:
                     log(f"t={time.perf_counter() - t0:.2f}s step of {len(members)} members "
@@ -165,7 +225,8 @@ class Server:
                     r.iters.append(int(it[j]))
                     r.eps.append(eps[j])
                     if st[j] == 0:
-                        self.newton_iters += int(it[j])
+                        with lock:
+                            self.newton_iters += int(it[j])
                     m[2] = k + 1
                     done = st[j] != 0 or m[2] >= r.steps
                     if st[j] != 0:
@@ -184,6 +245,9 @@ class Server:
             if batch is not None:
                 batch.close()
             for r, pt, k in members:
+                pt.close()
+            for th, r, pt in solos:
+                th.join()
                 pt.close()
             todo.put(None)
             worker.join(timeout=60)

@@ -77,3 +77,21 @@ def test_budget_limits_members():
     assert srv.peak_bytes <= budget
     assert max(srv.step_members) <= k_max
     assert all(r.failed < 0 and r.state is not None for r in out)
+
+
+def test_solo_requests_match_plain_path():
+    """Requests above the solo size run on their own thread and stream with
+    the direct FP64 path, overlapping the batch of the others: their results
+    are the plain path's, bit for bit; the batched ones still match alone."""
+    print("serving: solo and batched requests together", flush=True)
+    srv = Server(max_members=8, solo_min_elements=12 * 12, max_solo=2)
+    out = srv.run(requests())
+    assert srv.solo_count >= 1
+    for k, r in enumerate(out):
+        assert r.failed < 0 and r.state is not None
+        its, x, c = plain(SPECS[k], STEPS[k])
+        assert r.iters == its, k
+        if SPECS[k].n ** 2 >= 12 * 12:
+            assert np.array_equal(r.state[1], c) and np.array_equal(r.state[0], x), k
+        else:
+            assert np.linalg.norm(r.state[1] - c) / np.linalg.norm(c) <= 1e-8
