@@ -110,6 +110,8 @@ typedef struct {
   double diag_kernel_us;           /* mean duration of the diagonal-tile LU kernel
                                       (its middle launch in each factorization) */
   double last_relres;              /* |K du + r| / |r| of the last solve */
+  int32_t sweep_groups;            /* workgroups of each triangular sweep (autotuned) */
+  int32_t reserved;
 } mafem_stats;
 
 /* Builds the device context: uploads the tables, builds the block-sparse

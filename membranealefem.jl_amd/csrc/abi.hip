@@ -438,6 +438,35 @@ double solve_refined(Ctx &c, const double *d_b, double *d_x, int max_sweeps, dou
   return rel;
 }
 
+// Autotuner of the sweep split: on the first factorization of a context whose
+// sweeps run over several workgroups, time one band solve for each candidate
+// count (2, 4, 8, 16; at most one per four tile rows) and keep the fastest.
+// All candidates give bit-identical results (the per-row summation order does
+// not depend on the count), so the choice affects time only.  An
+// MAFEM_SWEEP_GROUPS setting disables the tuning.
+void tune_sweeps(Ctx &c, const double *d_b) {
+  c.sweep_tuned = true;
+  if (c.sweep_groups <= 1 || std::getenv("MAFEM_SWEEP_GROUPS")) return;
+  float best = 1e30f;
+  int best_w = c.sweep_groups;
+  for (int w : {2, 4, 8, 16}) {
+    if (w > c.nt / 4) break;
+    c.sweep_groups = w;
+    band_solve(c, d_b, c.d_t);  // warm
+    MAFEM_HIP(hipEventRecord(c.ev[6], c.stream));
+    band_solve(c, d_b, c.d_t);
+    MAFEM_HIP(hipEventRecord(c.ev[7], c.stream));
+    MAFEM_HIP(hipEventSynchronize(c.ev[7]));
+    float ms = 0.f;
+    MAFEM_HIP(hipEventElapsedTime(&ms, c.ev[6], c.ev[7]));
+    if (ms < best) {
+      best = ms;
+      best_w = w;
+    }
+  }
+  c.sweep_groups = best_w;
+}
+
 void refactor(Ctx &c) {
   band_fill(c, c.opts.factor_precision);
   band_factor(c);
@@ -451,6 +480,7 @@ double linear_solve(Ctx &c, const double *d_b, double *d_x) {
   MAFEM_HIP(hipEventRecord(c.ev[2], c.stream));
   if (o.mode == MAFEM_SOLVER_DIRECT) {
     refactor(c);
+    if (!c.sweep_tuned) tune_sweeps(c, d_b);
     MAFEM_HIP(hipEventRecord(c.ev[3], c.stream));
     const double rel = solve_refined(c, d_b, d_x, o.refine_sweeps, o.rtol);
     if (rel <= 1e3 * o.rtol) return rel;
@@ -467,6 +497,7 @@ double linear_solve(Ctx &c, const double *d_b, double *d_x) {
     refactor(c);
     fresh = true;
   }
+  if (!c.sweep_tuned) tune_sweeps(c, d_b);
   MAFEM_HIP(hipEventRecord(c.ev[3], c.stream));
   int its = 0;
   double rel = fgmres(c, d_b, d_x, &its);
@@ -732,6 +763,7 @@ int mafem_time_step(mafem_ctx *ctx, double time, double dt, int32_t max_iter, in
     st.n_factor = c.n_factor - nf0;
     st.gmres_iters = c.n_gmres - ng0;
     if (n_diag > 0) st.diag_kernel_us /= n_diag;
+    st.sweep_groups = c.sweep_groups;
     c.stats = st;
     if (iters) *iters = it;
     if (!(eps < c.enr)) {

@@ -131,6 +131,7 @@ struct Ctx {
 
   // triangular sweeps split over sweep_groups workgroups (1: one workgroup)
   int sweep_groups = 1;
+  bool sweep_tuned = false;      // the autotuner has chosen sweep_groups
   double *d_s1 = nullptr, *d_s2 = nullptr;  // sweep outputs (sentinel-filled, atomic access only)
   int *d_sweep_err = nullptr;    // count of sweep waits that ran out (must stay 0)
 

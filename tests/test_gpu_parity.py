@@ -284,3 +284,31 @@ def test_multi_workgroup_sweeps_larger_patches(motion, n, groups, monkeypatch):
         sols.append(x)
         dev.close()
     assert np.linalg.norm(sols[1] - sols[0]) / np.linalg.norm(sols[0]) <= 1e-8
+
+
+def test_sweep_autotuner_changes_time_only(monkeypatch):
+    """The sweep split is autotuned on the first factorization; every
+    candidate gives bit-identical results, so a time step with the tuned
+    split equals one with a fixed split."""
+    print("sweep autotuner", flush=True)
+    p = mafem.Params(motion=mafem.LAG, num1el=64, num2el=64, length=1.0, enr=1e-10, output=False)
+    mesh = mafem.make_mesh(p, pull_speed=0.3)
+    xms0, cps0 = mafem.prepare_input(p, mesh, pull_speed=0.3)
+    out = []
+    for forced in (None, "2"):
+        if forced is None:
+            monkeypatch.delenv("MAFEM_SWEEP_GROUPS", raising=False)
+        else:
+            monkeypatch.setenv("MAFEM_SWEEP_GROUPS", forced)
+        dev = mafem.DeviceMesh(mesh, p)
+        dev.set_state(xms0, cps0)
+        dev.predict(1e-3)
+        rc, eps = dev.time_step(1e-3, 1e-3)
+        assert rc == 0
+        g = dev.stats()["sweep_groups"]
+        print(f"  forced={forced}: sweep groups {g}", flush=True)
+        if forced is None:
+            assert g in (2, 4, 8, 16)
+        out.append(dev.get_state())
+        dev.close()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
