@@ -124,7 +124,7 @@ def bench_batched(args):
     if args.refactor_iters is None:
         args.refactor_iters = 6
     batch = mafem.Batch([pt.dev for pt in patches], precision=args.precision,
-                        refactor_iters=args.refactor_iters)
+                        refactor_iters=args.refactor_iters, use_graphs=not args.no_graphs)
     setup_s = time.perf_counter() - t0
     dt = 1e-3
 
@@ -220,7 +220,8 @@ def bench_batched(args):
         "config": {"workload": f"{len(specs)} independent F_PULL patches, n ~ U{{{args.nmin}.."
                                f"{args.nmax}}}, kb and zv log-uniform in [0.1, 10], pull ramped "
                                f"to U[0.2, 3] over 200 steps, motion U{{LAG, ALEVB}}, seed 0xC4, "
-                               f"flat initial state, dt=1e-3, one batch, HIP graph replay",
+                               f"flat initial state, dt=1e-3, one batch"
+                               f"{'' if args.no_graphs else ', HIP graph replay'}",
                    "patches": len(specs), "elements": int(numel.sum()),
                    "band_lu_precision": args.precision, "refactor_iters": args.refactor_iters,
                    "time_steps": args.steps, "newton_iters": iters},
@@ -410,6 +411,8 @@ def main():
     ap.add_argument("--req-steps", type=int, default=50)
     ap.add_argument("--max-members", type=int, default=64)
     ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="batches: launch directly instead of replaying recorded graphs")
     ap.add_argument("--refactor-iters", type=int, default=None,
                     help="batches: refactor after a solve needing more Arnoldi steps (-1: every "
                          "Newton iteration)")

@@ -128,20 +128,22 @@ PatchDev describe(const Ctx &c) {
   return d;
 }
 
-// Workgroups of each patch's sweeps: the longest chains get several (up to
-// 16, one per 64 tile rows) while the total stays within what the device
-// keeps resident at once (one 1024-thread sweep workgroup per CU), so every
-// workgroup a sweep waits on is running.
+// Workgroups of each patch's sweeps.  A split sweep is bound by the per-row
+// hand-off (~6 us) once each workgroup streams its rows faster than that:
+// 3 workgroups for FP32 storage, 4 for FP64 are enough.  The longest chains
+// (at least 96 tile rows) get a split first, while the total stays within
+// what the device keeps resident at once (one 1024-thread sweep workgroup per
+// CU), so every workgroup a sweep waits on is running.
 void assign_sweep_groups(std::vector<PatchDev> &pd, int budget) {
   std::vector<int> order(pd.size());
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return pd[x].nt > pd[y].nt; });
   for (int p : order) {
-    const int w = (int)std::min<int64_t>(16, pd[p].nt / 64);
-    if (w > 1 && w <= budget) {
-      pd[p].sweep_w = w;
-      budget -= w;
-    }
+    if (pd[p].nt < 96) break;
+    const int w = pd[p].prec == 64 ? 4 : 3;
+    if (w > budget) break;
+    pd[p].sweep_w = w;
+    budget -= w;
   }
 }
 
