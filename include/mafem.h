@@ -163,6 +163,17 @@ int mafem_time_step(mafem_ctx *ctx, double time, double dt, int32_t max_iter,
 
 int mafem_get_stats(const mafem_ctx *ctx, mafem_stats *out);
 
+/* Pull force of the device state (calc_pull_force, src/analysis/PullForce.jl:
+ * 61-80, with the adjacent elements and node maps of get_adj_maps,
+ * PullForce.jl:25-53, for poly = 2) and the mean position of the pulled
+ * element's 9 control points (src/Analysis.jl:82-84): f_pull[3] = sum over
+ * the pulled element's nodes of the velocity rows of the area-element
+ * residuals of the 25 elements around it (taken by element id, as the
+ * reference does), x_pull[3].  The pulled element is ceil(numel / 2)
+ * (get_pull_el_id, PullForce.jl:8-13).  MAFEM_EINVAL if an adjacent id falls
+ * outside the mesh (the reference would raise a BoundsError). */
+int mafem_pull_force(mafem_ctx *ctx, double *x_pull, double *f_pull);
+
 /* Selects the linear solver (no counterpart in the reference, which always
  * uses a sparse direct solve: this chooses how FiniteElement.jl:38 is done).
  * Defaults: DIRECT, FP64, restart 30, 200 iterations, refactor above 20,
@@ -234,6 +245,11 @@ int mafem_batch_time_step(mafem_batch *b, const double *time, const double *dt,
                           const uint8_t *active, int32_t predict, int32_t max_iter, int32_t *iters,
                           double *eps_hist, int32_t *status);
 int mafem_batch_get_stats(const mafem_batch *b, mafem_batch_stats *out);
+/* mafem_pull_force for every member, ordered after the batch's last step:
+ * out[6 p .. 6 p + 5] = x_pull[3], f_pull[3] of member p (no counterpart in
+ * the reference, which evaluates calc_pull_force per mesh,
+ * src/Analysis.jl:81-90). */
+int mafem_batch_pull_force(mafem_batch *b, double *out);
 
 #ifdef __cplusplus
 }

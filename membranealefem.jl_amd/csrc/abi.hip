@@ -56,7 +56,7 @@ void free_all(Ctx &c) {
                   c.d_red,  c.d_csc_src, c.d_csc_val, c.d_eq,     c.d_band,   c.d_linv,
                   c.d_uinv, c.d_piv,  c.d_V,       c.d_Z,       c.d_h,      c.d_coef,
                   c.d_mdot_part, c.d_t2, c.d_presc_idx, c.d_presc_val, c.d_s1, c.d_s2,
-                  c.d_sweep_err};
+                  c.d_sweep_err, c.d_pf_el, c.d_pf_slot, c.d_pf_rel, c.d_pf_out};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c.h_red) (void)hipHostFree(c.h_red);
@@ -64,6 +64,24 @@ void free_all(Ctx &c) {
     if (e) (void)hipEventDestroy(e);
   if (c.stream) (void)hipStreamDestroy(c.stream);
 }
+
+}  // namespace
+
+// Device copies of the pull-force maps and its output buffers (first use).
+// Callers check c.pf_err first; HIP failures throw HipError.
+void mafem::ensure_pull(Ctx &c) {
+  if (c.d_pf_out || !c.pf_err.empty()) return;
+  std::lock_guard<std::mutex> no_capture(capture_mutex());
+  const size_t n = c.h_pf_el.size();
+  MAFEM_HIP(hipMalloc((void **)&c.d_pf_el, sizeof(int32_t) * n));
+  MAFEM_HIP(hipMalloc((void **)&c.d_pf_slot, n * 9));
+  MAFEM_HIP(hipMalloc((void **)&c.d_pf_rel, sizeof(double) * n * c.ne));
+  MAFEM_HIP(hipMemcpy(c.d_pf_el, c.h_pf_el.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+  MAFEM_HIP(hipMemcpy(c.d_pf_slot, c.h_pf_slot.data(), n * 9, hipMemcpyHostToDevice));
+  MAFEM_HIP(hipMalloc((void **)&c.d_pf_out, sizeof(double) * 6));
+}
+
+namespace {
 
 int fail(Ctx &c, const std::string &m, int code) {
   c.err = m;
@@ -135,6 +153,42 @@ void build(Ctx &c, const mafem_mesh_desc *m, const mafem_params *p) {
     if (v < 1 || v > numnp) throw ArgError{"IX entry out of range", MAFEM_EINVAL};
     ix[k] = (int32_t)(v - 1);
   }
+  // pull-force maps (get_pull_el_id and get_adj_maps, PullForce.jl:8-53, with
+  // poly = 2): the 5 x 5 element ids around the pulled one taken by id
+  // arithmetic (rows wrap as in the reference), and for each the pairing of
+  // the pulled element's shared local nodes with its own, both in increasing
+  // local order
+  {
+    const int64_t pull = (numel + 1) / 2;  // ceil(numel / 2), 1-based
+    const int poly = 2;
+    c.pf_pull_el = (int32_t)(pull - 1);
+    const int32_t *pn = &ix[9 * (pull - 1)];
+    for (int i = -poly; i <= poly && c.pf_err.empty(); ++i)
+      for (int64_t e = pull + i * m->num1el - poly; e <= pull + i * m->num1el + poly; ++e) {
+        if (e < 1 || e > numel) {
+          c.pf_err = "pull force: adjacent element id " + std::to_string(e) +
+                     " is outside 1.." + std::to_string(numel);
+          break;
+        }
+        const int32_t *an = &ix[9 * (e - 1)];
+        int l1[9], l2[9], n1 = 0, n2 = 0;
+        for (int a = 0; a < 9; ++a)
+          for (int b = 0; b < 9; ++b)
+            if (pn[a] == an[b]) { l1[n1++] = a; break; }
+        for (int b = 0; b < 9; ++b)
+          for (int a = 0; a < 9; ++a)
+            if (an[b] == pn[a]) { l2[n2++] = b; break; }
+        c.h_pf_el.push_back((int32_t)(e - 1));
+        for (int b = 0; b < 9; ++b) c.h_pf_slot.push_back(-1);
+        for (int k = 0; k < n1 && k < n2; ++k)
+          c.h_pf_slot[c.h_pf_slot.size() - 9 + l2[k]] = (int8_t)l1[k];
+      }
+    if (!c.pf_err.empty()) {
+      c.h_pf_el.clear();
+      c.h_pf_slot.clear();
+    }
+  }
+
   std::vector<uint8_t> active(ndf * numnp);
   int64_t nmdf = 0;
   std::vector<int64_t> eq_ext;  // extended index of each equation
@@ -769,6 +823,24 @@ int mafem_time_step(mafem_ctx *ctx, double time, double dt, int32_t max_iter, in
     if (!(eps < c.enr)) {
       c.err = "did not reach Newton--Raphson tolerance";
       return MAFEM_NOT_CONVERGED;
+    }
+    return MAFEM_OK;
+  });
+}
+
+int mafem_pull_force(mafem_ctx *ctx, double *x_pull, double *f_pull) {
+  ABI_GUARD(ctx, {
+    if (!x_pull || !f_pull) return fail(c, "null output", MAFEM_EINVAL);
+    if (!c.pf_err.empty()) return fail(c, c.pf_err, MAFEM_EINVAL);
+    MAFEM_HIP(hipSetDevice(c.dev));
+    ensure_pull(c);
+    launch_pull_force(c, c.stream);
+    MAFEM_HIP(hipMemcpyAsync(c.h_red, c.d_pf_out, sizeof(double) * 6, hipMemcpyDeviceToHost,
+                             c.stream));
+    MAFEM_HIP(hipStreamSynchronize(c.stream));
+    for (int k = 0; k < 3; ++k) {
+      x_pull[k] = c.h_red[k];
+      f_pull[k] = c.h_red[3 + k];
     }
     return MAFEM_OK;
   });

@@ -91,14 +91,19 @@ __device__ __forceinline__ void geometry(const double *X /*[3][9]*/, const doubl
   g.n[2] = (g.a1[0] * g.a2[1] - g.a1[1] * g.a2[0]) * iJ;
 }
 
-template <int MO>
+// LIST = false: elements 0..numel-1, outputs K_el and r_el per element.
+// LIST = true: the numel elements elist[0..numel-1], output r_el only, at the
+// element's position in the list.
+template <int MO, bool LIST = false>
 __device__ __forceinline__ void area_body(
     int64_t bid, int64_t numel, int64_t numnp, const int32_t *__restrict__ ix,
     const int32_t *__restrict__ uel, const uint8_t *__restrict__ active,
     const double *__restrict__ xms, const double *__restrict__ cps,
     const double *__restrict__ ufns, const double *__restrict__ dbt, PhysParams ph,
-    double dt, double *__restrict__ kel, double *__restrict__ rel) {
+    double dt, double *__restrict__ kel, double *__restrict__ rel,
+    const int32_t *__restrict__ elist = nullptr) {
   using L = Lay<MO>;
+  auto EL = [&](int64_t i) -> int64_t { return LIST ? (int64_t)elist[i] : i; };
   constexpr int NDF = L::NDF, NE = L::NE, MAXE = L::MAXE;
   __shared__ double sX[MAXE][3 * 9];
   __shared__ double sC[MAXE][NDF * 9];
@@ -113,7 +118,7 @@ __device__ __forceinline__ void area_body(
   for (int k = threadIdx.x; k < nle * 9 * (3 + NDF); k += 256) {
     const int le = k / (9 * (3 + NDF)), rem = k % (9 * (3 + NDF));
     const int comp = rem / 9, a = rem % 9;
-    const int64_t node = ix[(e_lo + le) * 9 + a];
+    const int64_t node = ix[EL(e_lo + le) * 9 + a];
     if (comp < 3)
       sX[le][comp * 9 + a] = xms[node + numnp * comp];
     else
@@ -122,17 +127,18 @@ __device__ __forceinline__ void area_body(
   // stage the 9 Gauss-point basis records of each element
   for (int k = threadIdx.x; k < nle * 9 * RECP; k += 256) {
     const int le = k / (9 * RECP), rem = k % (9 * RECP);
-    sB[le][rem] = ufns[(int64_t)uel[e_lo + le] * 9 * RECP + rem];
+    sB[le][rem] = ufns[(int64_t)uel[EL(e_lo + le)] * 9 * RECP + rem];
   }
   __syncthreads();
 
   const int64_t f = f0 + threadIdx.x;
   const bool valid = f < numel * NE;
   const int64_t e = valid ? f / NE : e_hi;
+  const int64_t eg = EL(e);
   const int col = (int)(f % NE);
   const int le = (int)(e - e_lo);
   const int a0 = col / NDF, d0 = col % NDF;
-  const int64_t node0 = ix[e * 9 + a0];
+  const int64_t node0 = ix[eg * 9 + a0];
   const bool act = active[d0 + NDF * node0] != 0;
   const double sC0 = act ? 1.0 : 0.0;
   const bool mot = (L::MOT0 >= 0) && d0 >= L::MOT0 && d0 < L::MOT0 + 3;
@@ -350,7 +356,7 @@ __device__ __forceinline__ void area_body(
 
   // Dohrmann--Bochev projection (FiniteElement.jl:315-327): linear in λ, p^m
   {
-    const double *T = dbt + (int64_t)uel[e] * 81;
+    const double *T = dbt + (int64_t)uel[eg] * 81;
     if (d0 == L::LAMI && act) {
 #pragma unroll
       for (int a = 0; a < 9; ++a) acc[a * NDF + L::LAMI] += T[a * 9 + a0] * azv;
@@ -369,9 +375,11 @@ __device__ __forceinline__ void area_body(
   }
 
   if (valid) {
-    double *K = kel + e * (int64_t)NE * NE;
+    if constexpr (!LIST) {
+      double *K = kel + e * (int64_t)NE * NE;
 #pragma unroll
-    for (int row = 0; row < NE; ++row) K[row * NE + col] = acc[row];
+      for (int row = 0; row < NE; ++row) K[row * NE + col] = acc[row];
+    }
     rel[e * NE + col] = racc;
   }
 }
@@ -508,6 +516,46 @@ __global__ __launch_bounds__(256) void k_edge(
     double *__restrict__ kel, double *__restrict__ rel) {
   edge_body<MO>(blockIdx.x, n_items, numnp, items, ix, active, xms, cps, bufns, ph, time, dt,
                 kel, rel);
+}
+
+// Residuals of a list of area elements (the pull force's adjacent elements).
+template <int MO>
+__global__ __launch_bounds__(256) void k_area_list(
+    int64_t n, int64_t numnp, const int32_t *__restrict__ ix, const int32_t *__restrict__ uel,
+    const uint8_t *__restrict__ active, const double *__restrict__ xms,
+    const double *__restrict__ cps, const double *__restrict__ ufns,
+    const double *__restrict__ dbt, PhysParams ph, const int32_t *__restrict__ elist,
+    double *__restrict__ rel) {
+  area_body<MO, true>(blockIdx.x, n, numnp, ix, uel, active, xms, cps, ufns, dbt, ph, 0.0, nullptr,
+                      rel, elist);
+}
+
+// Pull force and pulled-element position (PullForce.jl:61-80,
+// Analysis.jl:81-85).  Threads 0-2: f[c] = sum over pulled nodes k of
+// (sum over adjacent elements, in order, of the v_c row of the adjacent
+// element's local node that is pulled node k).  Threads 3-5: mean of xms over
+// the pulled element's 9 nodes.
+__global__ void k_pull_reduce(int n, int ne, int ndf, const int8_t *__restrict__ slot,
+                              const double *__restrict__ rel, const int32_t *__restrict__ pull_ix,
+                              const double *__restrict__ xms, int64_t numnp,
+                              double *__restrict__ out) {
+  const int t = threadIdx.x;
+  if (t < 3) {
+    double f = 0.0;
+    for (int k = 0; k < 9; ++k) {
+      double s = 0.0;
+      for (int q = 0; q < n; ++q)
+        for (int a = 0; a < 9; ++a)
+          if (slot[q * 9 + a] == k) s += rel[(int64_t)q * ne + a * ndf + t];
+      f += s;
+    }
+    out[3 + t] = f;
+  } else if (t < 6) {
+    const int c = t - 3;
+    double s = 0.0;
+    for (int a = 0; a < 9; ++a) s += xms[pull_ix[a] + numnp * c];
+    out[c] = s / 9.0;
+  }
 }
 
 // Batched forms: block -> patch through the segment map; patches that are
@@ -686,6 +734,27 @@ int launch_assemble(Ctx &c, double time, double dt) {
                      c.nx, c.ndf, c.d_active, c.d_rptr, c.d_rsrc, c.d_rel, c.d_r);
   MAFEM_HIP(hipGetLastError());
   return 0;
+}
+
+void launch_pull_force(Ctx &c, hipStream_t s) {
+  const int64_t n = (int64_t)c.h_pf_el.size();
+  const dim3 grid((unsigned)((n * c.ne + 255) / 256));
+#define PLIST(MO)                                                                            \
+  hipLaunchKernelGGL(k_area_list<MO>, grid, dim3(256), 0, s, n, c.numnp, c.d_ix, c.d_uel,    \
+                     c.d_active, c.d_xms, c.d_cps, c.d_ufns, c.d_dbt, c.phys, c.d_pf_el,     \
+                     c.d_pf_rel)
+  switch (c.motion) {
+    case LAG: PLIST(LAG); break;
+    case EUL: PLIST(EUL); break;
+    case ALEV: PLIST(ALEV); break;
+    case ALEVB: PLIST(ALEVB); break;
+    default: MAFEM_HIP(hipErrorInvalidValue);
+  }
+#undef PLIST
+  MAFEM_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_pull_reduce, dim3(1), dim3(64), 0, s, (int)n, c.ne, c.ndf, c.d_pf_slot,
+                     c.d_pf_rel, c.d_ix + 9 * (int64_t)c.pf_pull_el, c.d_xms, c.numnp, c.d_pf_out);
+  MAFEM_HIP(hipGetLastError());
 }
 
 void launch_predict(Ctx &c, const double *d_u, double dt) {

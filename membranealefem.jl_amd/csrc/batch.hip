@@ -679,6 +679,34 @@ int mafem_batch_time_step(mafem_batch *bp, const double *time, const double *dt,
   });
 }
 
+int mafem_batch_pull_force(mafem_batch *bp, double *out) {
+  BATCH_GUARD(bp, {
+    if (!out) {
+      b.err = "null output";
+      return MAFEM_EINVAL;
+    }
+    const int P = (int)b.members.size();
+    for (int p = 0; p < P; ++p)
+      if (!b.members[p]->c.pf_err.empty()) {
+        b.err = "member " + std::to_string(p) + ": " + b.members[p]->c.pf_err;
+        return MAFEM_EINVAL;
+      }
+    MAFEM_HIP(hipSetDevice(b.dev));
+    // on the batch stream, after the members' last batched step
+    for (int p = 0; p < P; ++p) {
+      Ctx &c = b.members[p]->c;
+      ensure_pull(c);
+      launch_pull_force(c, b.B.stream);
+      MAFEM_HIP(hipMemcpyAsync(c.h_red, c.d_pf_out, sizeof(double) * 6, hipMemcpyDeviceToHost,
+                               b.B.stream));
+    }
+    MAFEM_HIP(hipStreamSynchronize(b.B.stream));
+    for (int p = 0; p < P; ++p)
+      for (int k = 0; k < 6; ++k) out[6 * p + k] = b.members[p]->c.h_red[k];
+    return MAFEM_OK;
+  });
+}
+
 int mafem_batch_get_stats(const mafem_batch *b, mafem_batch_stats *out) {
   if (!b || !out) return MAFEM_EINVAL;
   *out = b->stats;

@@ -139,6 +139,18 @@ struct Ctx {
   int64_t *d_presc_idx = nullptr;
   double *d_presc_val = nullptr;
   int64_t presc_cap = 0;
+
+  // pull force (PullForce.jl:8-80): the pulled element, its adjacent elements
+  // in the reference's order and, per adjacent element and local node, the
+  // local index in the pulled element of the same node (-1: not a pulled node)
+  int32_t pf_pull_el = -1;
+  std::vector<int32_t> h_pf_el;
+  std::vector<int8_t> h_pf_slot;
+  std::string pf_err;            // why the maps do not exist (empty: they do)
+  int32_t *d_pf_el = nullptr;    // device copies, allocated on first use
+  int8_t *d_pf_slot = nullptr;
+  double *d_pf_rel = nullptr;    // r_el of the adjacent elements [n][ne]
+  double *d_pf_out = nullptr;    // x_pull[3], f_pull[3]
 };
 
 // ---------------------------------------------------------------- batches
@@ -264,6 +276,10 @@ std::mutex &capture_mutex();
 
 // kernels / launchers (assemble.hip)
 int launch_assemble(Ctx &c, double time, double dt);
+// pulled-element mean position and pull force of the current state into
+// c.d_pf_out, on stream s (device tables from ensure_pull)
+void launch_pull_force(Ctx &c, hipStream_t s);
+void ensure_pull(Ctx &c);  // abi.hip: device copies of the pull-force maps
 void launch_predict(Ctx &c, const double *d_u, double dt);
 // band LU and solves (band.hip)
 void band_fill(Ctx &c, int prec);

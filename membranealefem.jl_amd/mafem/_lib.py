@@ -121,6 +121,8 @@ SIGNATURES = {
     "mafem_batch_time_step": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p, u8p, ctypes.c_int32,
                                              ctypes.c_int32, i32p, f64p, i32p]),
     "mafem_batch_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(BatchStats)]),
+    "mafem_pull_force": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p]),
+    "mafem_batch_pull_force": (ctypes.c_int, [ctypes.c_void_p, f64p]),
 }
 
 _lib = None

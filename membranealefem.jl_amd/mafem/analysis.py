@@ -12,8 +12,9 @@ import numpy as np
 import scipy.sparse as sps
 
 from . import _lib as L
-from .enums import BOTTOM, LEFT, RIGHT, TOP, Unknown
-from .mesh import m_motion_order, m_order, v_order
+from . import output as out
+from .enums import BOTTOM, F_PULL, LEFT, RIGHT, TOP, Unknown
+from .mesh import get_pull_el_id, m_motion_order, m_order, v_order
 from .params import Params
 
 _BDRY = (BOTTOM, RIGHT, TOP, LEFT)
@@ -165,6 +166,15 @@ class DeviceMesh:
         self._check(self.lib.mafem_get_solver(self.h, ctypes.byref(o)))
         return {k: getattr(o, k) for k, _ in L.SolverOpts._fields_}
 
+    def pull_force(self):
+        """(x_pull, f_pull) of the device state: the mean position of the
+        pulled element's control points and calc_pull_force
+        (PullForce.jl:61-80, Analysis.jl:82-85)."""
+        x = np.zeros(3)
+        f = np.zeros(3)
+        self._check(self.lib.mafem_pull_force(self.h, L.ptr(x, L.f64p), L.ptr(f, L.f64p)))
+        return x, f
+
     def stats(self):
         s = L.Stats()
         self._check(self.lib.mafem_get_stats(self.h, ctypes.byref(s)))
@@ -217,6 +227,15 @@ class Batch:
         if rc not in (L.OK, L.NOT_CONVERGED):
             raise L.LibError(rc, self.lib.mafem_batch_last_error(self.h).decode())
         return it, [list(eh[k, : it[k]]) for k in range(self.n)], st
+
+    def pull_force(self):
+        """Per member (x_pull[3], f_pull[3]) after the last batch step, as an
+        (n, 6) array (DeviceMesh.pull_force of every member)."""
+        out = np.zeros((self.n, 6))
+        rc = self.lib.mafem_batch_pull_force(self.h, L.ptr(out, L.f64p))
+        if rc != L.OK:
+            raise L.LibError(rc, self.lib.mafem_batch_last_error(self.h).decode())
+        return out
 
     def stats(self):
         s = L.BatchStats()
@@ -274,25 +293,73 @@ def time_step(mesh, xms, cps, time, dt, p, **args):
     xms[...] = x
     cps[...] = c
     if p.output:
-        with open(os.path.join(args["out_path"], args["out_file"]), "a") as io:
-            for k, e in enumerate(eps):
-                io.write(f"--> iteration {k + 1}: ε = {e}\n")
+        out.append(os.path.join(args["out_path"], args["out_file"]), out.iteration_lines(eps))
     if rc == L.NOT_CONVERGED:
         raise AssertionError("did not reach Newton--Raphson tolerance")
     return eps
+
+
+def get_adj_maps(num1el, numel, IX, poly=2):
+    """get_adj_maps (PullForce.jl:25-53): the 1-based ids of the (2 poly + 1)^2
+    elements around the pulled one, taken by id arithmetic as the reference
+    does, and per element the pair (rows of the pulled element's velocity
+    residual, rows of the adjacent element's) of their shared nodes, 0-based,
+    3 rows per node, local nodes in increasing order on both sides."""
+    ix = np.asarray(IX)
+    pull = get_pull_el_id(numel)
+    pull_nodes = set(int(v) for v in ix[:, pull - 1])
+    ids = [pull + i * num1el + j for i in range(-poly, poly + 1) for j in range(-poly, poly + 1)]
+    maps = []
+    for e in ids:
+        if not 1 <= e <= numel:
+            raise IndexError(f"adjacent element {e} outside 1..{numel}")
+        adj = [int(v) for v in ix[:, e - 1]]
+        rp = [3 * a + c for a in range(9) if int(ix[a, pull - 1]) in adj for c in range(3)]
+        ra = [3 * b + c for b in range(9) if adj[b] in pull_nodes for c in range(3)]
+        maps.append((rp, ra))
+    return ids, maps
+
+
+def calc_pull_force(mesh, xms, cps, adj_el_ids, adj_node_map, p, **args):
+    """calc_pull_force (PullForce.jl:61-80) on the GPU: the pull force of the
+    state (xms, cps).  The device evaluates get_adj_maps itself; maps other
+    than get_adj_maps(mesh.num1el, mesh.numel, mesh.IX, 2) are rejected."""
+    ids, maps = get_adj_maps(mesh.num1el, mesh.numel, mesh.IX, 2)
+    if list(adj_el_ids) != ids or [tuple(map(list, m)) for m in adj_node_map] != \
+            [tuple(m) for m in maps]:
+        raise ValueError("only the adjacency of get_adj_maps with poly = 2 is supported")
+    dev = device_for(mesh, p, **args)
+    dev.set_state(xms, cps)
+    return dev.pull_force()[1]
 
 
 def run_analysis(mesh, xms, cps, p, **args):
     """run_analysis! (Analysis.jl:17-102): predictor + Newton per time step,
     with the state resident on the GPU; returns the ε history of each step.
 
+    With p.output it writes the reference's files under args["out_path"]
+    (times.txt, t<id>-xms.txt / t<id>-cps.txt after every step, f-pull.txt
+    for F_PULL with the device-evaluated pull force, iteration lines in
+    args["out_file"]); the state is downloaded only for those files.
+
     An optional `pull_schedule(t_id, time) -> speed` keyword re-applies the
     F_PULL prescribed speed before each step (what a restart with a new
     `pull_speed` does in the reference, src/Input.jl:103-108).
     """
     dev = device_for(mesh, p, **args)
-    dts = list(args["dts"])
+    dts = list(args["dts"] if "dts" in args else args["Δts"])
+    t0_id = int(args.get("t0_id", 0))
     times = np.cumsum(dts) + args.get("t0", 0.0)
+    log = os.path.join(args["out_path"], args["out_file"]) if p.output else None
+    if p.output:
+        out.append(log, "\nStarting to run analysis...\n")
+        out.append(os.path.join(args["out_path"], "times.txt"), out.times_text(times, t0_id))
+    pulled = p.scenario == F_PULL
+    if pulled:
+        assert len(v_order(mesh.dofs)) == 3, "f_pull needs 3-D velocity"
+        append_run = "in_path" in args and args.get("out_path") == args["in_path"]
+        if p.output and not append_run:
+            out.append(os.path.join(args["out_path"], "f-pull.txt"), out.PULL_HEADER)
     sched = args.get("pull_schedule")
     pres = None
     if sched is not None:
@@ -303,18 +370,67 @@ def run_analysis(mesh, xms, cps, p, **args):
     dev.set_state(xms, cps)
     hist = []
     for k, dt in enumerate(dts):
+        if p.output:
+            out.append(log, f"\n-> Time {out.julia_float(times[k])}\n")
         if sched is not None:
             s = float(sched(k, float(times[k])))
             dev.set_prescribed([n for n, _ in pres], [d for _, d in pres], [s] * len(pres))
         dev.predict(dt)
         rc, eps = dev.time_step(float(times[k]), float(dt))
         hist.append(eps)
+        if p.output:
+            out.append(log, out.iteration_lines(eps))
         if rc == L.NOT_CONVERGED:
             x, c = dev.get_state()
             xms[...] = x
             cps[...] = c
             raise AssertionError("did not reach Newton--Raphson tolerance")
+        if p.output:
+            x, c = dev.get_state()
+            out.write_state(args["out_path"], k + 1 + t0_id, x, c)
+            if pulled:
+                xp, fp = dev.pull_force()
+                out.append(os.path.join(args["out_path"], "f-pull.txt"),
+                           out.pull_line(times[k], xp, fp))
+    if p.output:
+        out.append(log, "\nCompleted running analysis...\n\n")
     x, c = dev.get_state()
     xms[...] = x
     cps[...] = c
     return hist
+
+
+def solve(p, **args):
+    """solve (src/MembraneAleFem.jl:44-60): parameters checked, mesh and
+    initial state prepared, time steps run on the GPU; returns (mesh, xms, cps)."""
+    from .mesh import make_mesh, prepare_input
+    from .params import check_params
+    if "Δts" in args and "dts" not in args:
+        args["dts"] = args["Δts"]
+    check_params(p, **args)
+    mesh = make_mesh(p, **args)
+    xms, cps = prepare_input(p, mesh, **args)
+    run_analysis(mesh, xms, cps, p, **args)
+    return mesh, xms, cps
+
+
+def restart(p, args, **r_args):
+    """restart (src/MembraneAleFem.jl:75-107) with the previous run's Params
+    and keyword arguments given as objects (the reference deserializes them
+    from files written by Julia's Serialization, which is not restated here):
+    requires in_path, Δts (or dts), t0_id and t0; reads the state from
+    t<t0_id>-xms.txt / t<t0_id>-cps.txt under in_path unless in_xms / in_cps
+    are given; writes to in_path unless out_path is given."""
+    for k in ("in_path", "t0_id", "t0"):
+        assert k in r_args, f"\nneed to overwrite '{k}' during restart"
+    assert "Δts" in r_args or "dts" in r_args, "\nneed to overwrite 'Δts' during restart"
+    r = dict(r_args)
+    r.setdefault("in_cps", f"t{r['t0_id']}-cps.txt")
+    r.setdefault("in_xms", f"t{r['t0_id']}-xms.txt")
+    r.setdefault("out_path", r["in_path"])
+    merged = dict(args)
+    merged.update(r)
+    if "Δts" in r:
+        merged["dts"] = r["Δts"]
+    return solve(p, **merged)
+

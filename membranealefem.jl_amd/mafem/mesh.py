@@ -9,6 +9,7 @@ memory layouts (1-based Int64, column-major), so they cross the C ABI as they
 would from Julia.
 """
 import math
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -203,7 +204,7 @@ def m_motion_order(motion, dofs):
 
 
 def get_pull_el_id(numel):
-    """Pulled element: ceil(numel / 2) (Input.jl:131-136)."""
+    """Pulled element: ceil(numel / 2) (get_pull_el_id, PullForce.jl:8-13)."""
     return (numel + 1) // 2
 
 
@@ -410,8 +411,15 @@ def prepare_input(p, mesh, **args):
     they are used instead (the restart path, Input.jl:35-37).
     """
     if "in_xms" in args and "in_cps" in args:
-        xms = np.array(args["in_xms"], dtype=np.float64, order="F")
-        cps = np.array(args["in_cps"], dtype=np.float64, order="F")
+        ix, ic = args["in_xms"], args["in_cps"]
+        if isinstance(ix, str):  # file names under in_path, read as readdlm does
+            from .output import readdlm
+            ix = readdlm(os.path.join(args["in_path"], ix))
+            ic = readdlm(os.path.join(args["in_path"], ic))
+        xms = np.array(ix, dtype=np.float64, order="F")
+        cps = np.array(ic, dtype=np.float64, order="F")
+        if xms.shape != (mesh.numnp, 3) or cps.shape != (mesh.numnp, mesh.ndf):
+            raise ValueError("restart state does not match the mesh")
     else:
         assert p.scenario != F_BEND, "F_BEND initial velocities are random in the reference"
         L = p.length
@@ -426,4 +434,7 @@ def prepare_input(p, mesh, **args):
         cps[node - 1, mesh.dofs[unk] - 1] = val
         if p.scenario == F_PULL and p.motion == EUL:
             cps[node - 1, mesh.dofs[Unknown.vmz] - 1] = val
+    if p.output:  # initial condition files (Input.jl:111-114)
+        from .output import write_state
+        write_state(args["out_path"], int(args.get("t0_id", 0)), xms, cps)
     return xms, cps

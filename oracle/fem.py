@@ -8,6 +8,7 @@ Follows (reference paths):
       eps = |du|_2 / nmdf, AssertionError on failure)
   src/analysis/FiniteElement.jl:408-423 update_xms
   src/Analysis.jl:17-102                run_analysis (predictor, then Newton)
+  src/analysis/PullForce.jl:8-80        get_adj_maps, calc_pull_force
 
 The reference solves with UMFPACK (SuiteSparse through SparseArrays `\\`); this
 oracle uses scipy's SuperLU, so the solution agrees to rounding only: the
@@ -16,7 +17,12 @@ linear-solve rounding is "parity unpinned" against UMFPACK itself.
 Pinned by: the equilibrium invariant (flat F_PULL state at t = 0 with zero
 pull speed gives r = 0), central finite differences of the residual against
 the tangent, quadratic Newton convergence, and the closed-form GeoDynStress
-tests of the reference (tests/test_oracle_*.py).
+tests of the reference (tests/test_oracle_*.py).  The pull force is pinned by
+its closed form on the flat tension state (f = (kb/4) L x the integral of the
+pulled element's edge basis functions for an edge element, 0 for an interior
+one), by the sum of every element's residual over the pulled nodes, and by
+the force curve of the reference's documented tether example
+(docs/src/index.md:142-176, example-pull.png).
 """
 import ctypes
 import os
@@ -236,6 +242,56 @@ def run_analysis(om_, xms, cps, dts, t0=0.0, bend_tm=1.0, max_iter=14):
         hist.append(time_step(om_, xms, cps, float(times[k]), float(dt),
                               max_iter=max_iter, bend_tm=bend_tm))
     return hist
+
+
+def get_adj_maps(num1el, numel, IX, poly=2):
+    """get_adj_maps (PullForce.jl:25-53): the (2 poly + 1)^2 element ids
+    around the pulled one, by id arithmetic (1-based), and per element the
+    pair (rows of rv_pull, rows of rv_el) of the pulled element's nodes it
+    shares: local indices in increasing order on both sides, 3 rows each."""
+    pull = om.get_pull_el_id(numel)
+    pull_nodes = IX[:, pull - 1]
+    ids = [pull + i * num1el + j for i in range(-poly, poly + 1)
+           for j in range(-poly, poly + 1)]
+    maps = []
+    for e in ids:
+        adj_nodes = IX[:, e - 1]
+        rows_pull = [3 * a + c for a in range(9) if pull_nodes[a] in adj_nodes
+                     for c in range(3)]
+        rows_adj = [3 * b + c for b in range(9) if adj_nodes[b] in pull_nodes
+                    for c in range(3)]
+        maps.append((rows_pull, rows_adj))
+    return ids, maps
+
+
+def elem_rv(om_, el, xms, cps):
+    """rv of calc_elem_dof_residuals (FiniteElement.jl:253-330) for one area
+    element (0-based), layout comp + 3 (node - 1), from the oracle residual."""
+    m = om_.mesh
+    X = np.asfortranarray(xms, dtype=np.float64)
+    C = np.asfortranarray(cps, dtype=np.float64)
+    r = np.zeros(9 * m.ndf)
+    lib().orc_area_residual(ctypes.byref(om_.s), _ip(om_.uel), _dp(om_.ufns), int(el),
+                            _dp(X), _dp(C), _dp(r))
+    vo = om.v_order(m.dofs)
+    return np.array([r[(vo[c] - 1) + m.ndf * a] for a in range(9) for c in range(3)])
+
+
+def calc_pull_force(om_, xms, cps, adj_el_ids, adj_node_map):
+    """calc_pull_force (PullForce.jl:61-80): the 3-vector sum over the pulled
+    nodes of the velocity residual rows of the adjacent elements."""
+    rv_pull = np.zeros(27)
+    for k, e in enumerate(adj_el_ids):
+        rv = elem_rv(om_, e - 1, xms, cps)
+        rp, ra = adj_node_map[k]
+        rv_pull[rp] += rv[ra]
+    return rv_pull.reshape(9, 3).sum(axis=0)
+
+
+def pull_position(mesh, xms):
+    """Mean position of the pulled element's 9 nodes (Analysis.jl:82-84)."""
+    nodes = mesh.IX[:, om.get_pull_el_id(mesh.numel) - 1] - 1
+    return xms[nodes, :].sum(axis=0) / 9.0
 
 
 def build(p, **args):

@@ -93,7 +93,7 @@ def m_motion_order(motion, dofs):
 
 
 def get_pull_el_id(numel):
-    """get_pull_el_id (Input.jl:131-136): ceil(numel/2)."""
+    """get_pull_el_id (PullForce.jl:8-13): ceil(numel/2)."""
     return -(-numel // 2)
 
 
