@@ -65,6 +65,10 @@ typedef struct {
   const int32_t *neu_bdry;         /* Boundary enum, 1..4 */
   const int32_t *neu_type;         /* Neumann enum: SHEAR 1, STRETCH 2, MOMENT 3 */
   const double *neu_val;
+  /* mesh.crnr_gp_fns (Mesh.jl:228-233): 4 records of 55 Float64 in Corner
+   * order BOTTOM_LEFT, BOTTOM_RIGHT, TOP_LEFT, TOP_RIGHT (Enums.jl:109-114);
+   * NULL makes mafem_generate_output unavailable */
+  const double *crnr_ufns;
 } mafem_mesh_desc;
 
 /* The hot-path subset of `Params` (src/input/Params.jl:36-55) plus the
@@ -173,6 +177,15 @@ int mafem_get_stats(const mafem_ctx *ctx, mafem_stats *out);
  * (get_pull_el_id, PullForce.jl:8-13).  MAFEM_EINVAL if an adjacent id falls
  * outside the mesh (the reference would raise a BoundsError). */
 int mafem_pull_force(mafem_ctx *ctx, double *x_pull, double *f_pull);
+
+/* generate_output(mesh, xms, cps) (src/Output.jl:30-118) of the device
+ * state: the surface position and the unknowns sampled at every area Gauss
+ * point, at the 3 Gauss points of each boundary element on the four edges and
+ * at the four corners, on a (3 num1el + 2) x (3 num2el + 2) grid.  xout is
+ * that grid x 3 and uout that grid x ndf, both column-major like the
+ * reference's Array{Float64,3}; either may be NULL.  MAFEM_EINVAL when the
+ * corner basis tables were not given at create time. */
+int mafem_generate_output(mafem_ctx *ctx, double *xout, double *uout);
 
 /* Selects the linear solver (no counterpart in the reference, which always
  * uses a sparse direct solve: this chooses how FiniteElement.jl:38 is done).

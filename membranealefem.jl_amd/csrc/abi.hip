@@ -56,7 +56,8 @@ void free_all(Ctx &c) {
                   c.d_red,  c.d_csc_src, c.d_csc_val, c.d_eq,     c.d_band,   c.d_linv,
                   c.d_uinv, c.d_piv,  c.d_V,       c.d_Z,       c.d_h,      c.d_coef,
                   c.d_mdot_part, c.d_t2, c.d_presc_idx, c.d_presc_val, c.d_s1, c.d_s2,
-                  c.d_sweep_err, c.d_pf_el, c.d_pf_slot, c.d_pf_rel, c.d_pf_out};
+                  c.d_sweep_err, c.d_pf_el, c.d_pf_slot, c.d_pf_rel, c.d_pf_out,
+                  c.d_obel, c.d_crnr, c.d_out};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c.h_red) (void)hipHostFree(c.h_red);
@@ -267,6 +268,30 @@ void build(Ctx &c, const mafem_mesh_desc *m, const mafem_params *p) {
         std::memcpy(&bufns[off], src, sizeof(double) * REC);
       }
   }
+  // output sampling tables: boundary elements with their first records, and
+  // the corner records (corner elements 1, num1el, numel - num1el + 1, numel,
+  // Mesh.jl:134-139)
+  for (int b = 0; b < 4; ++b) {
+    c.h_boff[b] = (int64_t)c.h_obel.size() / 2;
+    for (int64_t j = 0; j < m->bdry_n_el[b]; ++j) {
+      const int64_t el = m->bdry_elems[b][j], u = m->bdry_uel_ids[b][j];
+      if (el < 1 || el > numel || u < 1 || u > m->bdry_n_uel[b])
+        throw ArgError{"boundary table entry out of range", MAFEM_EINVAL};
+      c.h_obel.push_back((int32_t)(el - 1));
+      c.h_obel.push_back((int32_t)(base[b] + (u - 1) * 3));
+    }
+  }
+  c.h_boff[4] = (int64_t)c.h_obel.size() / 2;
+  if (m->crnr_ufns) {
+    c.h_crnr.assign(4 * RECP, 0.0);
+    for (int k = 0; k < 4; ++k)
+      std::memcpy(&c.h_crnr[k * RECP], m->crnr_ufns + REC * k, sizeof(double) * REC);
+  }
+  c.crnr_el[0] = 0;
+  c.crnr_el[1] = (int32_t)(c.num1el - 1);
+  c.crnr_el[2] = (int32_t)(numel - c.num1el);
+  c.crnr_el[3] = (int32_t)(numel - 1);
+
   std::vector<EdgeItem> edges;
   for (int64_t k = 0; k < m->n_neu; ++k) {
     const int b = m->neu_bdry[k] - 1;
@@ -842,6 +867,33 @@ int mafem_pull_force(mafem_ctx *ctx, double *x_pull, double *f_pull) {
       x_pull[k] = c.h_red[k];
       f_pull[k] = c.h_red[3 + k];
     }
+    return MAFEM_OK;
+  });
+}
+
+int mafem_generate_output(mafem_ctx *ctx, double *xout, double *uout) {
+  ABI_GUARD(ctx, {
+    if (c.h_crnr.empty()) return fail(c, "corner basis tables (crnr_ufns) not given", MAFEM_EINVAL);
+    const int64_t want[4] = {c.num1el, c.num2el, c.num1el, c.num2el};
+    for (int b = 0; b < 4; ++b)
+      if (c.h_boff[b + 1] - c.h_boff[b] != want[b])
+        return fail(c, "boundary element lists do not cover the edges", MAFEM_EINVAL);
+    MAFEM_HIP(hipSetDevice(c.dev));
+    const int64_t npts = (3 * c.num1el + 2) * (3 * c.num2el + 2);
+    if (!c.d_out) {
+      std::lock_guard<std::mutex> no_capture(capture_mutex());
+      c.d_obel = dupload(c, c.h_obel);
+      c.d_crnr = dupload(c, c.h_crnr);
+      c.d_out = dalloc<double>(c, npts * (3 + c.ndf));
+    }
+    launch_output(c);
+    if (xout)
+      MAFEM_HIP(hipMemcpyAsync(xout, c.d_out, sizeof(double) * npts * 3, hipMemcpyDeviceToHost,
+                               c.stream));
+    if (uout)
+      MAFEM_HIP(hipMemcpyAsync(uout, c.d_out + npts * 3, sizeof(double) * npts * c.ndf,
+                               hipMemcpyDeviceToHost, c.stream));
+    MAFEM_HIP(hipStreamSynchronize(c.stream));
     return MAFEM_OK;
   });
 }

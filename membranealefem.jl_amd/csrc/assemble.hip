@@ -558,6 +558,60 @@ __global__ void k_pull_reduce(int n, int ne, int ndf, const int8_t *__restrict__
   }
 }
 
+// generate_output (Output.jl:30-118): one thread per sample (i1, i2) of the
+// Z1 x Z2 grid (0-based).  Interior samples are the area Gauss points
+// (i = 3 (el - 1) + gp, shifted by one), the outer rows/columns the boundary
+// Gauss points of BOTTOM (i2 = 0), RIGHT (i1 = Z1 - 1), TOP (i2 = Z2 - 1) and
+// LEFT (i1 = 0) in boundary-element order, the four corners the corner
+// records.  Each sample is Σ_a state[node_a] N_a over the element's 9 nodes.
+struct OutGrid {
+  int64_t Z1, Z2, num1el, numnp;
+  int32_t ndf;
+  int32_t crnr_el[4];
+  int64_t boff[4];
+};
+
+__global__ __launch_bounds__(256) void k_generate_output(
+    OutGrid g, const int32_t *__restrict__ ix, const int32_t *__restrict__ uel,
+    const double *__restrict__ ufns, const double *__restrict__ bufns,
+    const int32_t *__restrict__ obel, const double *__restrict__ crnr,
+    const double *__restrict__ xms, const double *__restrict__ cps, double *__restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t npts = g.Z1 * g.Z2;
+  if (t >= npts) return;
+  const int64_t i1 = t % g.Z1, i2 = t / g.Z1;
+  const bool e1 = (i1 == 0 || i1 == g.Z1 - 1), e2 = (i2 == 0 || i2 == g.Z2 - 1);
+  int64_t el;
+  const double *N;
+  if (!e1 && !e2) {
+    const int64_t el1 = (i1 - 1) / 3, el2 = (i2 - 1) / 3;
+    const int gp = (int)((i1 - 1) % 3 + 3 * ((i2 - 1) % 3));
+    el = el1 + g.num1el * el2;
+    N = ufns + ((int64_t)uel[el] * 9 + gp) * RECP + 1;
+  } else if (e1 && e2) {
+    const int k = (i1 == 0 ? 0 : 1) + (i2 == 0 ? 0 : 2);
+    el = g.crnr_el[k];
+    N = crnr + k * RECP + 1;
+  } else {
+    int b;
+    int64_t k;
+    if (i2 == 0) { b = 0; k = i1 - 1; }
+    else if (i1 == g.Z1 - 1) { b = 1; k = i2 - 1; }
+    else if (i2 == g.Z2 - 1) { b = 2; k = i1 - 1; }
+    else { b = 3; k = i2 - 1; }
+    const int32_t *pr = obel + 2 * (g.boff[b] + k / 3);
+    el = pr[0];
+    N = bufns + ((int64_t)pr[1] + k % 3) * RECP + 1;
+  }
+  const int32_t *nodes = ix + el * 9;
+  for (int c = 0; c < 3 + g.ndf; ++c) {
+    const double *src = c < 3 ? xms + g.numnp * c : cps + g.numnp * (c - 3);
+    double s = 0.0;
+    for (int a = 0; a < 9; ++a) s = fma(src[nodes[a]], N[a], s);
+    out[t + npts * c] = s;
+  }
+}
+
 // Batched forms: block -> patch through the segment map; patches that are
 // off skip the launch.
 template <int MO>
@@ -754,6 +808,24 @@ void launch_pull_force(Ctx &c, hipStream_t s) {
   MAFEM_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_pull_reduce, dim3(1), dim3(64), 0, s, (int)n, c.ne, c.ndf, c.d_pf_slot,
                      c.d_pf_rel, c.d_ix + 9 * (int64_t)c.pf_pull_el, c.d_xms, c.numnp, c.d_pf_out);
+  MAFEM_HIP(hipGetLastError());
+}
+
+void launch_output(Ctx &c) {
+  OutGrid g;
+  g.Z1 = 3 * c.num1el + 2;
+  g.Z2 = 3 * c.num2el + 2;
+  g.num1el = c.num1el;
+  g.numnp = c.numnp;
+  g.ndf = c.ndf;
+  for (int k = 0; k < 4; ++k) {
+    g.crnr_el[k] = c.crnr_el[k];
+    g.boff[k] = c.h_boff[k];
+  }
+  const int64_t npts = g.Z1 * g.Z2;
+  hipLaunchKernelGGL(k_generate_output, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0,
+                     c.stream, g, c.d_ix, c.d_uel, c.d_ufns, c.d_bufns, c.d_obel, c.d_crnr,
+                     c.d_xms, c.d_cps, c.d_out);
   MAFEM_HIP(hipGetLastError());
 }
 

@@ -151,6 +151,16 @@ struct Ctx {
   int8_t *d_pf_slot = nullptr;
   double *d_pf_rel = nullptr;    // r_el of the adjacent elements [n][ne]
   double *d_pf_out = nullptr;    // x_pull[3], f_pull[3]
+
+  // output sampling (Output.jl:30-118): per boundary, its elements in order as
+  // (0-based element, first record in d_bufns) pairs from h_obel[2 h_boff[b]];
+  // corner records (RECP doubles each) and their elements
+  std::vector<int32_t> h_obel;
+  int64_t h_boff[5] = {};
+  std::vector<double> h_crnr;    // empty: not given
+  int32_t crnr_el[4] = {};
+  int32_t *d_obel = nullptr;
+  double *d_crnr = nullptr, *d_out = nullptr;
 };
 
 // ---------------------------------------------------------------- batches
@@ -280,6 +290,9 @@ int launch_assemble(Ctx &c, double time, double dt);
 // c.d_pf_out, on stream s (device tables from ensure_pull)
 void launch_pull_force(Ctx &c, hipStream_t s);
 void ensure_pull(Ctx &c);  // abi.hip: device copies of the pull-force maps
+// generate_output sampling of the device state into c.d_out: xout
+// (Z1 x Z2 x 3) then uout (Z1 x Z2 x ndf), column-major, Zα = 3 numαel + 2
+void launch_output(Ctx &c);
 void launch_predict(Ctx &c, const double *d_u, double dt);
 // band LU and solves (band.hip)
 void band_fill(Ctx &c, int prec);

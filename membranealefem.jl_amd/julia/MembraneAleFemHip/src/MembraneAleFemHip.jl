@@ -1,8 +1,8 @@
 # MembraneAleFemHip: runs MembraneAleFem.jl's per-time-step hot path
 # (calc_r_K + K \ r inside time_step!) on an MI355X through libmafem's C ABI
 # (include/mafem.h).  Loading this package after MembraneAleFem replaces
-# `time_step!` and `calc_r_K`; `solve`, `restart`, `prepare_input` and
-# `run_analysis!` stay the reference's own.
+# `time_step!`, `calc_r_K` and `calc_pull_force`; `solve`, `restart`,
+# `prepare_input` and `run_analysis!` stay the reference's own.
 #
 #   using MembraneAleFem, MembraneAleFemHip
 #   MembraneAleFemHip.enable!()            # route time_step!/calc_r_K to the GPU
@@ -32,6 +32,7 @@ struct MeshDesc
   bdry_n_uel::NTuple{4,Int64}; bdry_uel_ids::NTuple{4,Ptr{Int64}}
   bdry_ufns::NTuple{4,Ptr{Float64}}
   n_neu::Int64; neu_bdry::Ptr{Int32}; neu_type::Ptr{Int32}; neu_val::Ptr{Float64}
+  crnr_ufns::Ptr{Float64}
 end
 
 # include/mafem.h: mafem_params
@@ -55,9 +56,11 @@ mutable struct Context
     neu_v = Float64[v for (_, _, v) in mesh.inh_neu_bcs]
     belems = [mesh.bdry_elems[b] for b in bdrys]
     bfns = [mesh.bdry_gp_fns[b] for b in bdrys]
+    crnr = [mesh.crnr_gp_fns[c]
+            for c in (MAF.BOTTOM_LEFT, MAF.BOTTOM_RIGHT, MAF.TOP_LEFT, MAF.TOP_RIGHT)]
     keep = Any[mesh.IX, mesh.ID, mesh.area_gp_fns.uel_ids, mesh.area_gp_fns.ufns,
                belems..., [f.uel_ids for f in bfns]..., [f.ufns for f in bfns]...,
-               neu_b, neu_t, neu_v]
+               neu_b, neu_t, neu_v, crnr]
     GC.@preserve keep begin
       d = MeshDesc(mesh.num1el, mesh.num2el, mesh.numel, mesh.num1np, mesh.num2np, mesh.numnp,
                    mesh.ndf, mesh.nmdf, pointer(mesh.IX), pointer(mesh.ID),
@@ -70,7 +73,8 @@ mutable struct Context
                    Tuple(Int64(size(f.ufns, 1)) for f in bfns),
                    Tuple(pointer(f.uel_ids) for f in bfns),
                    Tuple(Ptr{Float64}(pointer(f.ufns)) for f in bfns),
-                   length(neu_b), pointer(neu_b), pointer(neu_t), pointer(neu_v))
+                   length(neu_b), pointer(neu_b), pointer(neu_t), pointer(neu_v),
+                   Ptr{Float64}(pointer(crnr)))
       q = ParamsDesc(Int32(p.motion), Int32(p.scenario), p.kb, p.kg, p.ζv, p.pn, p.αdb, p.αm,
                      p.εk, p.εnr, bend_tm)
       hr = Ref{Ptr{Cvoid}}(C_NULL)
@@ -136,7 +140,29 @@ function time_step!(mesh, xms, cps, time, Δt, p; args...)
   return Nothing
 end
 
-"Route MembraneAleFem's time_step!/calc_r_K to the GPU (call after loading)."
+"calc_pull_force on the GPU (PullForce.jl:61-80); the device builds the same
+adjacent elements and node maps as get_adj_maps with poly = 2."
+function calc_pull_force(mesh, xms, cps, adj_el_ids, adj_node_map, p)
+  ctx = context(mesh, p)
+  set_state!(ctx, xms, cps)
+  x = zeros(3); f = zeros(3)
+  check(ctx, ccall((:mafem_pull_force, libmafem), Cint,
+                   (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}), ctx.h, x, f))
+  return f
+end
+
+"generate_output (Output.jl:30-118) on the GPU, for a mesh that has a context."
+function generate_output(mesh, xms, cps)
+  ctx = CONTEXTS[mesh]
+  set_state!(ctx, xms, cps)
+  n1 = mesh.num1el * 3 + 2; n2 = mesh.num2el * 3 + 2
+  xout = zeros(n1, n2, 3); uout = zeros(n1, n2, mesh.ndf)
+  check(ctx, ccall((:mafem_generate_output, libmafem), Cint,
+                   (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}), ctx.h, xout, uout))
+  return xout, uout
+end
+
+"Route MembraneAleFem's time_step!/calc_r_K/calc_pull_force to the GPU (call after loading)."
 function enable!()
   @eval MAF function time_step!(mesh::Mesh, xms::Matrix{Float64}, cps::Matrix{Float64},
                                 time::Float64, Δt::Float64, p::Params; args...)
@@ -145,6 +171,12 @@ function enable!()
   @eval MAF function calc_r_K(mesh::Mesh, xms::Matrix{Float64}, cps::Matrix{Float64},
                               time::Float64, Δt::Float64, p::Params; args...)
     return $(calc_r_K)(mesh, xms, cps, time, Δt, p; args...)
+  end
+  @eval MAF function calc_pull_force(mesh::Mesh, xms::Matrix{Float64}, cps::Matrix{Float64},
+                                     adj_el_ids::Vector{Int64},
+                                     adj_node_map::Vector{Tuple{Vector{Int64},Vector{Int64}}},
+                                     p::Params)::Vector{Float64}
+    return $(calc_pull_force)(mesh, xms, cps, adj_el_ids, adj_node_map, p)
   end
   return nothing
 end

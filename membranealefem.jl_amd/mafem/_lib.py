@@ -32,6 +32,7 @@ class MeshDesc(ctypes.Structure):
         ("bdry_n_uel", ctypes.c_int64 * 4), ("bdry_uel_ids", i64p * 4),
         ("bdry_ufns", f64p * 4),
         ("n_neu", ctypes.c_int64), ("neu_bdry", i32p), ("neu_type", i32p), ("neu_val", f64p),
+        ("crnr_ufns", f64p),
     ]
 
 
@@ -122,6 +123,7 @@ SIGNATURES = {
                                              ctypes.c_int32, i32p, f64p, i32p]),
     "mafem_batch_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(BatchStats)]),
     "mafem_pull_force": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p]),
+    "mafem_generate_output": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p]),
     "mafem_batch_pull_force": (ctypes.c_int, [ctypes.c_void_p, f64p]),
 }
 

@@ -63,6 +63,8 @@ class DeviceMesh:
         keep += [nb, nt, nv]
         d.n_neu = len(nb)
         d.neu_bdry, d.neu_type, d.neu_val = L.ptr(nb, L.i32p), L.ptr(nt, L.i32p), L.ptr(nv, L.f64p)
+        if m.crnr_ufns is not None:
+            d.crnr_ufns = af(np.ascontiguousarray(m.crnr_ufns))
         q = L.ParamsDesc(int(p.motion), int(p.scenario), p.kb, p.kg, p.zv, p.pn, p.adb, p.am,
                          p.ek, p.enr, bend_tm)
         h = ctypes.c_void_p()
@@ -174,6 +176,16 @@ class DeviceMesh:
         f = np.zeros(3)
         self._check(self.lib.mafem_pull_force(self.h, L.ptr(x, L.f64p), L.ptr(f, L.f64p)))
         return x, f
+
+    def generate_output(self):
+        """generate_output (Output.jl:30-118) of the device state: (xout,
+        uout) of shapes (3 num1el + 2, 3 num2el + 2, 3) and (..., ndf)."""
+        m = self.mesh
+        shape = (3 * m.num1el + 2, 3 * m.num2el + 2)
+        x = np.zeros(shape + (3,), order="F")
+        u = np.zeros(shape + (m.ndf,), order="F")
+        self._check(self.lib.mafem_generate_output(self.h, L.ptr(x, L.f64p), L.ptr(u, L.f64p)))
+        return x, u
 
     def stats(self):
         s = L.Stats()
@@ -398,6 +410,21 @@ def run_analysis(mesh, xms, cps, p, **args):
     xms[...] = x
     cps[...] = c
     return hist
+
+
+def generate_output(mesh, xms, cps, p=None, **args):
+    """generate_output(mesh, xms, cps) (Output.jl:30-118) on the GPU: positions
+    and unknowns at the area Gauss points, the edge Gauss points and the
+    corners, as arrays (3 num1el + 2, 3 num2el + 2, 3) and (..., ndf).  Uses
+    the mesh's cached context (or one built from `p`)."""
+    if p is None:
+        if mesh._ctx is None:
+            raise ValueError("no context on this mesh: pass the Params")
+        dev = mesh._ctx[1]
+    else:
+        dev = device_for(mesh, p, **args)
+    dev.set_state(xms, cps)
+    return dev.generate_output()
 
 
 def solve(p, **args):

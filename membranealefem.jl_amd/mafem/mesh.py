@@ -233,6 +233,7 @@ class Mesh:
     nmdf: int
     inh_dir_bcs: list
     inh_neu_bcs: list
+    crnr_ufns: np.ndarray = None        # (4, 55): Corner order (Mesh.jl:228-233)
     _ctx: object = field(default=None, repr=False, compare=False)
 
     @property
@@ -305,9 +306,11 @@ def make_mesh(p, **args):
     ID = np.zeros((ndf, nn), dtype=np.int64, order="F")
     flat = free.ravel(order="F")                      # node-major, DOF fastest
     ID.ravel(order="F")[:] = np.where(flat, np.cumsum(flat), 0)
+    crnr_ufns = np.array([tensor_record(l1.zmin, l2.zmin), tensor_record(l1.zmax, l2.zmin),
+                          tensor_record(l1.zmin, l2.zmax), tensor_record(l1.zmax, l2.zmax)])
     return Mesh(n1e, n2e, numel, n1, n2, nn, ix, bdry_elems, bdry_nodes, bdry_inner, crnr_nodes,
                 kv1, kv2, area_uel, area, bdry_uel, bdry_ufns, dofs, ndf, ID, int(ID.max()),
-                dir_bcs, neu_bcs)
+                dir_bcs, neu_bcs, crnr_ufns)
 
 
 def _scenario_bcs(p, nn, ix, bn, bin_, crnr, **args):
@@ -402,8 +405,39 @@ def collocation_cps_1d(knots, f):
     return np.linalg.solve(A, np.array([f(zz) for zz in z]))
 
 
+def _bend_initial_state(p, mesh, **args):
+    """F_BEND initial state (Input.jl:44-75): x = L ζ1, y = L ζ2, z = 0;
+    velocities (and mesh velocities unless LAG) drawn uniformly, vx, vy from
+    (U - 1/2) / numel and vz from U / numel, then zero on the boundary nodes;
+    λ = kb / 4 / L^2 (Δt_1 / bend_tm)^2.  The reference draws with Julia's
+    unseeded rand(); here the draws come from numpy's default_rng(bend_seed)
+    (bend_seed = 0 unless given), in the same order."""
+    rng = np.random.default_rng(int(args.get("bend_seed", 0)))
+    L = p.length
+    nn, ne = mesh.numnp, mesh.numel
+    c1 = collocation_cps_1d(mesh.kv1, lambda z: L * z)
+    c2 = collocation_cps_1d(mesh.kv2, lambda z: L * z)
+    xms = np.zeros((nn, 3), order="F")
+    xms[:, 0] = np.tile(c1, mesh.num2np)
+    xms[:, 1] = np.repeat(c2, mesh.num1np)
+    cps = np.zeros((nn, mesh.ndf), order="F")
+    d = mesh.dofs
+    cps[:, d[Unknown.vx] - 1] = (rng.random(nn) - 0.5) / ne
+    cps[:, d[Unknown.vy] - 1] = (rng.random(nn) - 0.5) / ne
+    cps[:, d[Unknown.vz] - 1] = rng.random(nn) / ne
+    if p.motion != LAG:
+        cps[:, d[Unknown.vmx] - 1] = (rng.random(nn) - 0.5) / ne
+        cps[:, d[Unknown.vmy] - 1] = (rng.random(nn) - 0.5) / ne
+        cps[:, d[Unknown.vmz] - 1] = rng.random(nn) / ne
+    for b in (BOTTOM, RIGHT, TOP, LEFT):
+        cps[np.asarray(mesh.bdry_nodes[b]) - 1, :] = 0.0
+    dts = args["dts"] if "dts" in args else args["Δts"]
+    cps[:, d[Unknown.lam] - 1] = p.kb / 4 / L ** 2 * (dts[0] / args["bend_tm"]) ** 2
+    return xms, cps
+
+
 def prepare_input(p, mesh, **args):
-    """prepare_input (Input.jl:22-121) for the deterministic scenarios.
+    """prepare_input (Input.jl:22-121).
 
     Positions fit x = L(ζ1 - 1/2), y = L(ζ2 - 1/2), z = 0 (the reference's
     2-D collocation solve factorises into these 1-D solves); λ = kb/4; then
@@ -420,8 +454,9 @@ def prepare_input(p, mesh, **args):
         cps = np.array(ic, dtype=np.float64, order="F")
         if xms.shape != (mesh.numnp, 3) or cps.shape != (mesh.numnp, mesh.ndf):
             raise ValueError("restart state does not match the mesh")
+    elif p.scenario == F_BEND:
+        xms, cps = _bend_initial_state(p, mesh, **args)
     else:
-        assert p.scenario != F_BEND, "F_BEND initial velocities are random in the reference"
         L = p.length
         c1 = collocation_cps_1d(mesh.kv1, lambda z: L * (z - 0.5))
         c2 = collocation_cps_1d(mesh.kv2, lambda z: L * (z - 0.5))

@@ -9,6 +9,7 @@ Follows (reference paths):
   src/analysis/FiniteElement.jl:408-423 update_xms
   src/Analysis.jl:17-102                run_analysis (predictor, then Newton)
   src/analysis/PullForce.jl:8-80        get_adj_maps, calc_pull_force
+  src/Output.jl:30-118                  generate_output
 
 The reference solves with UMFPACK (SuiteSparse through SparseArrays `\\`); this
 oracle uses scipy's SuperLU, so the solution agrees to rounding only: the
@@ -292,6 +293,44 @@ def pull_position(mesh, xms):
     """Mean position of the pulled element's 9 nodes (Analysis.jl:82-84)."""
     nodes = mesh.IX[:, om.get_pull_el_id(mesh.numel) - 1] - 1
     return xms[nodes, :].sum(axis=0) / 9.0
+
+
+def generate_output(mesh, xms, cps):
+    """generate_output (Output.jl:30-118): state sampled at the area Gauss
+    points, the boundary Gauss points (BOTTOM, RIGHT, TOP, LEFT rows and
+    columns, in boundary-element order) and the corners."""
+    z1, z2 = mesh.num1el * 3 + 2, mesh.num2el * 3 + 2
+    xout = np.zeros((z1, z2, 3))
+    uout = np.zeros((z1, z2, mesh.ndf))
+    for e2 in range(1, mesh.num2el + 1):
+        for e1 in range(1, mesh.num1el + 1):
+            el = e1 + mesh.num1el * (e2 - 1)
+            nodes = mesh.IX[:, el - 1] - 1
+            for g2 in range(1, 4):
+                for g1 in range(1, 4):
+                    gp = g1 + 3 * (g2 - 1)
+                    N = mesh.area_ufns[mesh.area_uel_ids[el - 1] - 1, gp - 1, 1:10]
+                    o1, o2 = (e1 - 1) * 3 + g1, (e2 - 1) * 3 + g2  # 0-based of +1
+                    xout[o1, o2] = xms[nodes].T @ N
+                    uout[o1, o2] = cps[nodes].T @ N
+    for b in (om.BOTTOM, om.RIGHT, om.TOP, om.LEFT):
+        uids, ufns = mesh.bdry_fns[b]
+        out_id = 1
+        for j, el in enumerate(mesh.bdry_elems[b]):
+            nodes = mesh.IX[:, el - 1] - 1
+            for g in range(3):
+                N = ufns[uids[j] - 1, g, 1:10]
+                o1, o2 = {om.BOTTOM: (out_id, 0), om.RIGHT: (z1 - 1, out_id),
+                          om.TOP: (out_id, z2 - 1), om.LEFT: (0, out_id)}[b]
+                xout[o1, o2] = xms[nodes].T @ N
+                uout[o1, o2] = cps[nodes].T @ N
+                out_id += 1
+    for k, (o1, o2) in enumerate([(0, 0), (z1 - 1, 0), (0, z2 - 1), (z1 - 1, z2 - 1)]):
+        nodes = mesh.IX[:, mesh.crnr_elems[k] - 1] - 1
+        N = mesh.crnr_fns[k][1:10]
+        xout[o1, o2] = xms[nodes].T @ N
+        uout[o1, o2] = cps[nodes].T @ N
+    return xout, uout
 
 
 def build(p, **args):

@@ -183,6 +183,9 @@ def make_mesh(p, **args):
         TOP: sp.bdry_basis(l1, l2.zmax, True),
         LEFT: sp.bdry_basis(l2, l1.zmin, False),
     }
+    m.crnr_fns = [sp.gp_basis_2d(l1.zmin, l2.zmin), sp.gp_basis_2d(l1.zmax, l2.zmin),
+                  sp.gp_basis_2d(l1.zmin, l2.zmax), sp.gp_basis_2d(l1.zmax, l2.zmax)]
+    m.crnr_elems = [1, n1e, ne - n1e + 1, ne]
     m.IX = construct_IX(m.kv1, m.kv2, n1)
     _generate_scenario(m, p, **args)
     return m
@@ -326,16 +329,31 @@ def prepare_input(p, mesh, **args):
     """prepare_input (Input.jl:22-121), non-random scenarios only.
 
     Positions are the collocation fit of x = L(ζ1 - 1/2), y = L(ζ2 - 1/2),
-    z = 0; λ = kb/4; then the inhomogeneous Dirichlet values.  F_BEND's
-    unseeded random velocities (Input.jl:44-75) are not restated.
+    z = 0; λ = kb/4; then the inhomogeneous Dirichlet values.  F_BEND
+    (Input.jl:44-75): x = L ζ1, y = L ζ2, seeded uniform velocities.
     """
-    assert p.scenario != F_BEND, "F_BEND initial state is random in the reference"
     xms = np.zeros((mesh.numnp, 3))
     cps = np.zeros((mesh.numnp, mesh.ndf))
     L = p.length
-    xms[:, 0] = sp.get_2d_bspline_cps(mesh.kv1, mesh.kv2, lambda a, b: L * (a - 0.5))
-    xms[:, 1] = sp.get_2d_bspline_cps(mesh.kv1, mesh.kv2, lambda a, b: L * (b - 0.5))
-    cps[:, mesh.dofs[LAM] - 1] = p.kb / 4
+    if p.scenario == F_BEND:
+        # Input.jl:44-75, with Julia's unseeded rand() replaced by
+        # numpy's default_rng(bend_seed) drawn in the same order
+        rng = np.random.default_rng(int(args.get("bend_seed", 0)))
+        nn, ne = mesh.numnp, mesh.numel
+        xms[:, 0] = sp.get_2d_bspline_cps(mesh.kv1, mesh.kv2, lambda a, b: L * a)
+        xms[:, 1] = sp.get_2d_bspline_cps(mesh.kv1, mesh.kv2, lambda a, b: L * b)
+        cols = [VX, VY, VZ] + ([] if p.motion == LAG else [VMX, VMY, VMZ])
+        for u in cols:
+            shift = 0.0 if u in (VZ, VMZ) else 0.5
+            cps[:, mesh.dofs[u] - 1] = (rng.random(nn) - shift) / ne
+        for nodes in mesh.bdry_nodes.values():
+            cps[np.asarray(nodes) - 1, :] = 0.0
+        dts = args["dts"] if "dts" in args else args["Δts"]
+        cps[:, mesh.dofs[LAM] - 1] = p.kb / 4 / L ** 2 * (dts[0] / args["bend_tm"]) ** 2
+    else:
+        xms[:, 0] = sp.get_2d_bspline_cps(mesh.kv1, mesh.kv2, lambda a, b: L * (a - 0.5))
+        xms[:, 1] = sp.get_2d_bspline_cps(mesh.kv1, mesh.kv2, lambda a, b: L * (b - 0.5))
+        cps[:, mesh.dofs[LAM] - 1] = p.kb / 4
     for (unk, node, val) in mesh.inh_dir_bcs:
         cps[node - 1, mesh.dofs[unk] - 1] = val
         if p.scenario == F_PULL and p.motion == EUL:

@@ -166,3 +166,22 @@ def test_batch_pull_force_matches_single():
     finally:
         for d in devs:
             d.close()
+
+
+@pytest.mark.parametrize("motion,n1,n2", [(mafem.LAG, 4, 3), (mafem.ALEVB, 20, 19),
+                                          (mafem.EUL, 9, 9), (mafem.ALEVB, 64, 64)])
+def test_generate_output_matches_oracle(motion, n1, n2):
+    print(f"generate_output motion={motion.name} {n1}x{n2}", flush=True)
+    p = mafem.Params(motion=motion, num1el=n1, num2el=n2, length=2.0, output=False)
+    mesh = mafem.make_mesh(p, pull_speed=0.3)
+    xms, cps = mafem.prepare_input(p, mesh, pull_speed=0.3)
+    rng = np.random.default_rng(n1 + 100 * n2)
+    xms += rng.normal(0, 1e-2, xms.shape)
+    cps += rng.normal(0, 1e-1, cps.shape)
+    xout, uout = mafem.generate_output(mesh, xms, cps, p)
+    po = om.Params(motion=int(motion), num1el=n1, num2el=n2, length=2.0)
+    mo = om.make_mesh(po, pull_speed=0.3)
+    xo, uo = fem.generate_output(mo, xms, cps)
+    assert xout.shape == xo.shape and uout.shape == uo.shape
+    assert np.abs(xout - xo).max() <= 1e-14 * np.abs(xo).max()
+    assert np.abs(uout - uo).max() <= 1e-14 * np.abs(uo).max()

@@ -79,3 +79,20 @@ def test_host_adjacency_rejects_tiny_meshes():
     m = mafem.make_mesh(p, pull_speed=0.0)
     with pytest.raises(IndexError):
         mafem.get_adj_maps(m.num1el, m.numel, m.IX)
+
+
+@pytest.mark.parametrize("n1,n2", [(4, 3), (20, 19)])
+def test_host_corner_records_match_oracle(n1, n2):
+    po = om.Params(motion=om.ALEVB, scenario=om.F_PULL, num1el=n1, num2el=n2, output=False)
+    mo = om.make_mesh(po, pull_speed=0.0)
+    p = mafem.Params(motion=mafem.ALEVB, scenario=mafem.F_PULL, num1el=n1, num2el=n2,
+                     output=False)
+    m = mafem.make_mesh(p, pull_speed=0.0)
+    assert m.crnr_ufns.shape == (4, 55)
+    for k in range(4):   # N exactly; derivatives to rounding (second derivatives ~1e3)
+        assert np.array_equal(m.crnr_ufns[k][:10], mo.crnr_fns[k][:10])
+        assert np.allclose(m.crnr_ufns[k], mo.crnr_fns[k], rtol=1e-14, atol=0)
+    # corner basis of clamped knots: 1 at the corner node, 0 elsewhere
+    local = [0, 2, 6, 8]
+    for k in range(4):
+        assert m.crnr_ufns[k][1 + local[k]] == 1.0 and np.abs(m.crnr_ufns[k][1:10]).sum() == 1.0

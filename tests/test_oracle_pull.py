@@ -1,4 +1,6 @@
-"""Oracle pull force (PullForce.jl:8-80), pinned three ways.
+"""Oracle pull force (PullForce.jl:8-80), pinned three ways, and the oracle's
+generate_output (Output.jl:30-118), pinned by exact sampling of the linear
+and constant fields of the flat state.
 
 * Closed form on the flat F_PULL state: with x = L (ζ - 1/2), uniform tension
   λ = kb/4 and velocities normal to the plane, the velocity residual of a node
@@ -117,3 +119,35 @@ def test_documented_tether_force_curve():
         fz = fem.calc_pull_force(o, xms, cps, ids, maps)[2]
         assert abs(zp - zf) < 1e-9
         assert abs(fz - ff) < 0.03, (k, zp, fz, ff)
+
+
+def _samples(kv):
+    """ζ samples of generate_output along one direction: 0, the 3 Gauss
+    abscissae of every element, 1."""
+    t = kv.zs
+    xi = (-np.sqrt(0.6), 0.0, np.sqrt(0.6))
+    out = [0.0]
+    for e in range(kv.nel):
+        lo, hi = t[e + 2], t[e + 3]
+        out += [0.5 * (lo + hi) + 0.5 * (hi - lo) * x for x in xi]
+    return np.array(out + [1.0])
+
+
+@pytest.mark.parametrize("motion,n1,n2,length", [(om.LAG, 4, 3, 1.0), (om.ALEVB, 20, 19, 2.0),
+                                                 (om.EUL, 1, 2, 1.0)])
+def test_generate_output_samples_linear_fields_exactly(motion, n1, n2, length):
+    """generate_output (Output.jl:30-118) on the flat state: B-splines
+    reproduce x = L (ζ - 1/2) and the constant λ, so every sample equals the
+    field at its ζ (Gauss points of each element, edges, corners)."""
+    p = om.Params(motion=motion, scenario=om.F_PULL, num1el=n1, num2el=n2, length=length,
+                  kb=0.8, output=False)
+    o, xms, cps = fem.build(p, pull_speed=0.0)
+    m = o.mesh
+    xout, uout = fem.generate_output(m, xms, cps)
+    z1, z2 = _samples(m.kv1), _samples(m.kv2)
+    assert xout.shape == (3 * n1 + 2, 3 * n2 + 2, 3)
+    assert uout.shape == (3 * n1 + 2, 3 * n2 + 2, m.ndf)
+    assert np.allclose(xout[:, :, 0], length * (z1[:, None] - 0.5), rtol=0, atol=1e-14 * length)
+    assert np.allclose(xout[:, :, 1], length * (z2[None, :] - 0.5), rtol=0, atol=1e-14 * length)
+    assert np.all(xout[:, :, 2] == 0.0)
+    assert np.allclose(uout[:, :, m.dofs[om.LAM] - 1], 0.2, rtol=0, atol=1e-15)
