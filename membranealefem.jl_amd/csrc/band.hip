@@ -202,24 +202,29 @@ __global__ __launch_bounds__(64) void k_diag(int64_t K, int tb, S *__restrict__ 
 
 // Panel of step K, one wavefront per tile.  kind 0: U_KJ = L_KK^-1 P_K A_KJ
 // (lane = column, forward substitution); kind 1: L_IK = A_IK U_KK^-1 (lane =
-// row, substitution against U, the tile staged through LDS for coalescing);
-// kind 2: P_K applied to the L tiles already stored left of the diagonal in
-// tile row K.  The factored diagonal tile is staged in LDS and read with
-// wave-uniform (broadcast) addresses.
+// row, substitution against U); kind 2: P_K applied to the L tiles already
+// stored left of the diagonal in tile row K.  The factored diagonal tile is
+// staged in LDS and read with wave-uniform (broadcast) addresses; the tile
+// being solved lives in LDS and is processed in blocks of 8 unknowns by loops
+// whose bodies are small (a fully unrolled 64 x 64 substitution is ~9 000
+// straight-line instructions, and fetching them dominated the kernel).  Every
+// entry receives its updates in increasing order of the eliminated index, as
+// in the unblocked substitution, so the results are the same bit for bit.
 template <class S>
 __device__ __forceinline__ void panel_body(int64_t bid, int64_t K, int64_t nt, int tb, S *__restrict__ band,
                                               const int32_t *__restrict__ piv) {
+  constexpr int B = 8;
   __shared__ S sD[T][T];
-  __shared__ S sA[T][T + 1];
+  __shared__ S sX[T][T + 1];
   __shared__ S rdiag[T];
   const int lane = threadIdx.x;
   const int kind = bid / tb, off = bid % tb;
   const int32_t *pv = piv + K * T;
-  S x[T];
   if (kind == 2) {
     const int64_t J = K - tb + off;
     if (J < 0) return;
     S *A = tile_ptr(band, tb, K, J);
+    S x[T];
 #pragma unroll
     for (int i = 0; i < T; ++i) x[i] = A[(int64_t)pv[i] * T + lane];
 #pragma unroll
@@ -233,37 +238,69 @@ __device__ __forceinline__ void panel_body(int64_t bid, int64_t K, int64_t nt, i
   for (int i = 0; i < T; ++i) sD[i][lane] = D[i * T + lane];
   if (kind == 0) {
     S *A = tile_ptr(band, tb, K, IJ);
-#pragma unroll
-    for (int i = 0; i < T; ++i) x[i] = A[(int64_t)pv[i] * T + lane];
+#pragma unroll 8
+    for (int i = 0; i < T; ++i) sX[i][lane] = A[(int64_t)pv[i] * T + lane];
     __syncthreads();
-    // right-looking: once x_k is final it is eliminated from every later row,
-    // so the updates of one step are independent of each other
+    // x_i -= D_ik x_k for k < i, k increasing (unit lower L)
+#pragma unroll 1
+    for (int b = 0; b < T / B; ++b) {
+      S xb[B];
 #pragma unroll
-    for (int k = 0; k < T - 1; ++k)
+      for (int r = 0; r < B; ++r) xb[r] = sX[B * b + r][lane];
+#pragma unroll 1
+      for (int c = 0; c < b; ++c) {
+        S xc[B];
 #pragma unroll
-      for (int i = k + 1; i < T; ++i) x[i] = fma(-sD[i][k], x[k], x[i]);
+        for (int q = 0; q < B; ++q) xc[q] = sX[B * c + q][lane];
 #pragma unroll
-    for (int i = 0; i < T; ++i) A[i * T + lane] = x[i];
+        for (int q = 0; q < B; ++q)
+#pragma unroll
+          for (int r = 0; r < B; ++r) xb[r] = fma(-sD[B * b + r][B * c + q], xc[q], xb[r]);
+      }
+#pragma unroll
+      for (int q = 0; q < B - 1; ++q)
+#pragma unroll
+        for (int r = q + 1; r < B; ++r) xb[r] = fma(-sD[B * b + r][B * b + q], xb[q], xb[r]);
+#pragma unroll
+      for (int r = 0; r < B; ++r) sX[B * b + r][lane] = xb[r];
+    }
+#pragma unroll 8
+    for (int i = 0; i < T; ++i) A[i * T + lane] = sX[i][lane];
   } else {
     S *A = tile_ptr(band, tb, IJ, K);
 #pragma unroll 8
-    for (int i = 0; i < T; ++i) sA[i][lane] = A[i * T + lane];
+    for (int i = 0; i < T; ++i) sX[i][lane] = A[i * T + lane];
     __syncthreads();
     rdiag[lane] = S(1) / sD[lane][lane];
-#pragma unroll
-    for (int k = 0; k < T; ++k) x[k] = sA[lane][k];
     __syncthreads();
+    // row `lane`: x_k -= x_j D_jk for j < k, j increasing; x_k *= 1 / D_kk
+#pragma unroll 1
+    for (int b = 0; b < T / B; ++b) {
+      S xb[B];
 #pragma unroll
-    for (int j = 0; j < T; ++j) {
-      x[j] *= rdiag[j];
+      for (int r = 0; r < B; ++r) xb[r] = sX[lane][B * b + r];
+#pragma unroll 1
+      for (int c = 0; c < b; ++c) {
+        S xc[B];
 #pragma unroll
-      for (int k = j + 1; k < T; ++k) x[k] = fma(-x[j], sD[j][k], x[k]);
+        for (int q = 0; q < B; ++q) xc[q] = sX[lane][B * c + q];
+#pragma unroll
+        for (int q = 0; q < B; ++q)
+#pragma unroll
+          for (int r = 0; r < B; ++r) xb[r] = fma(-xc[q], sD[B * c + q][B * b + r], xb[r]);
+      }
+#pragma unroll
+      for (int q = 0; q < B; ++q) {
+        xb[q] *= rdiag[B * b + q];
+#pragma unroll
+        for (int r = q + 1; r < B; ++r) xb[r] = fma(-xb[q], sD[B * b + q][B * b + r], xb[r]);
+      }
+#pragma unroll
+      for (int r = 0; r < B; ++r) sX[lane][B * b + r] = xb[r];
     }
-#pragma unroll
-    for (int k = 0; k < T; ++k) sA[lane][k] = x[k];
     __syncthreads();
 #pragma unroll 8
-    for (int i = 0; i < T; ++i) A[i * T + lane] = sA[i][lane];
+    for (int i = 0; i < T; ++i) A[i * T + lane] = sX[i][lane];
   }
 }
 
@@ -273,54 +310,103 @@ __global__ __launch_bounds__(64) void k_panel(int64_t K, int64_t nt, int tb, S *
   panel_body<S>((int64_t)blockIdx.x, K, nt, tb, band, piv);
 }
 
-// C = A B for 64x64 tiles staged in LDS; each of 256 threads owns a 4x4 block.
-template <class S>
-__device__ __forceinline__ void tile_mm(const S (*sa)[T], const S (*sb)[T], S c[4][4]) {
-  const int i0 = (threadIdx.x / 16) * 4, j0 = (threadIdx.x % 16) * 4;
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) c[a][b] = 0;
-#pragma unroll 8
-  for (int k = 0; k < T; ++k) {
-    S av[4], bv[4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a) av[a] = sa[i0 + a][k];
-#pragma unroll
-    for (int b = 0; b < 4; ++b) bv[b] = sb[k][j0 + b];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int b = 0; b < 4; ++b) c[a][b] = fma(av[a], bv[b], c[a][b]);
+// C = A B for row-major 64x64 tiles in global memory, on the matrix cores
+// (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32).  Both tiles are staged in
+// LDS with 16-byte coalesced loads (rows padded to 65 entries, so fragment
+// reads hit at most two lanes per bank); the 4 wavefronts of a 256-thread
+// block take 16 rows each.  For each 16x16 output block and MFMA step kk,
+// lane l (group g = l/16) feeds A[r0 + l%16][16g + kk] and
+// B[16g + kk][c0 + l%16]: over the 16 steps the four groups cover k = 0..63.
+// Result j = 0..3 of lane l lands at C[r0 + 4(l/16) + j][c0 + l%16] (FP32)
+// or C[r0 + l/16 + 4j][c0 + l%16] (FP64), layouts checked on the device.
+typedef double mm_d4 __attribute__((ext_vector_type(4)));
+typedef float mm_f4 __attribute__((ext_vector_type(4)));
+template <class S> struct Mfma;
+template <> struct Mfma<double> {
+  using V = mm_d4;
+  static __device__ __forceinline__ V run(double a, double b, V c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+};
+template <> struct Mfma<float> {
+  using V = mm_f4;
+  static __device__ __forceinline__ V run(float a, float b, V c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+// row / column of c[bj][j] in the 64x64 product
+template <class S>
+__device__ __forceinline__ int mm_row(int j) {
+  const int q = (int)((threadIdx.x & 63) >> 4);
+  return 16 * (int)(threadIdx.x >> 6) + (sizeof(S) == 8 ? q + 4 * j : 4 * q + j);
+}
+template <class S>
+__device__ __forceinline__ int mm_col(int bj) { return 16 * bj + (int)(threadIdx.x & 15); }
+
+template <class S>
+__device__ __forceinline__ void tile_mm(const S *__restrict__ a, const S *__restrict__ b,
+                                        S c[4][4]) {
+  using V = typename Mfma<S>::V;
+  constexpr int P = T + 1;
+  constexpr int W = 16 / sizeof(S);  // entries per 16-byte load
+  __shared__ S sa[T * P];
+  __shared__ S sb[T * P];
+  struct alignas(16) Vec { S v[W]; };
+  Vec va[T * T / W / 256], vb[T * T / W / 256];
+#pragma unroll
+  for (int it = 0; it < T * T / W / 256; ++it) {
+    const int e = (it * 256 + (int)threadIdx.x) * W;
+    va[it] = *(const Vec *)(a + e);
+    vb[it] = *(const Vec *)(b + e);
+  }
+#pragma unroll
+  for (int it = 0; it < T * T / W / 256; ++it) {
+    const int e = (it * 256 + (int)threadIdx.x) * W;
+    const int r = e / T, col = e % T;
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+      sa[r * P + col + q] = va[it].v[q];
+      sb[r * P + col + q] = vb[it].v[q];
+    }
+  }
+  __syncthreads();
+  const int w = (int)(threadIdx.x >> 6), l = (int)(threadIdx.x & 63);
+  const S *ar = sa + (16 * w + (l & 15)) * P + 16 * (l >> 4);
+  const S *br = sb + (16 * (l >> 4)) * P + (l & 15);
+  V acc[4];
+#pragma unroll
+  for (int bj = 0; bj < 4; ++bj) acc[bj] = V{0, 0, 0, 0};
+#pragma unroll
+  for (int kk = 0; kk < 16; ++kk) {
+    const S av = ar[kk];
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj) acc[bj] = Mfma<S>::run(av, br[kk * P + 16 * bj], acc[bj]);
+  }
+#pragma unroll
+  for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) c[bj][j] = acc[bj][j];
 }
 
 template <class S>
 __device__ __forceinline__ void update_body(int64_t bid, int64_t K, int64_t nt, int tb, S *__restrict__ band) {
-  __shared__ S sa[T][T];
-  __shared__ S sb[T][T];
   const int64_t I = K + 1 + bid / tb, J = K + 1 + bid % tb;
   if (I >= nt || J >= nt) return;
   const S *Lt = tile_ptr(band, tb, I, K);
   const S *Ut = tile_ptr(band, tb, K, J);
   S *A = tile_ptr(band, tb, I, J);
-  const int i0 = (threadIdx.x / 16) * 4, j0 = (threadIdx.x % 16) * 4;
   S old[4][4];  // the target tile is fetched together with the operands
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int bj = 0; bj < 4; ++bj)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) old[a][b] = A[(i0 + a) * T + j0 + b];
-  for (int k = threadIdx.x; k < T * T; k += 256) {
-    sa[k / T][k % T] = Lt[k];
-    sb[k / T][k % T] = Ut[k];
-  }
-  __syncthreads();
+    for (int j = 0; j < 4; ++j) old[bj][j] = A[mm_row<S>(j) * T + mm_col<S>(bj)];
   S c[4][4];
-  tile_mm<S>(sa, sb, c);
+  tile_mm<S>(Lt, Ut, c);
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int bj = 0; bj < 4; ++bj)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) A[(i0 + a) * T + j0 + b] = old[a][b] - c[a][b];
+    for (int j = 0; j < 4; ++j) A[mm_row<S>(j) * T + mm_col<S>(bj)] = old[bj][j] - c[bj][j];
 }
 
 template <class S>
@@ -379,26 +465,18 @@ template <class S>
 __device__ __forceinline__ void fold_body(int64_t bid, int64_t nt, int tb, S *__restrict__ band,
                                               const S *__restrict__ linv,
                                               const S *__restrict__ uinv) {
-  __shared__ S sa[T][T];
-  __shared__ S sb[T][T];
   const int64_t I = bid / (2 * tb + 1);
   const int slot = bid % (2 * tb + 1);
   const int64_t J = I - tb + slot;
   if (slot == tb || J < 0 || J >= nt) return;
   const S *Inv = (slot < tb ? linv : uinv) + I * (int64_t)T * T;
   S *A = tile_ptr(band, tb, I, J);
-  for (int k = threadIdx.x; k < T * T; k += 256) {
-    sa[k / T][k % T] = Inv[k];
-    sb[k / T][k % T] = A[k];
-  }
-  __syncthreads();
   S c[4][4];
-  tile_mm<S>(sa, sb, c);
-  const int i0 = (threadIdx.x / 16) * 4, j0 = (threadIdx.x % 16) * 4;
+  tile_mm<S>(Inv, A, c);  // reads all of A before its barrier
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int bj = 0; bj < 4; ++bj)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) A[(i0 + a) * T + j0 + b] = c[a][b];
+    for (int j = 0; j < 4; ++j) A[mm_row<S>(j) * T + mm_col<S>(bj)] = c[bj][j];
 }
 
 template <class S>
