@@ -533,9 +533,9 @@ __device__ __forceinline__ double tile_row4(const S *M, const double *v) {
 // results are used.
 template <class S, int B = (sizeof(S) == 4 ? 24 : 10)>  // loads in flight within the VGPR budget
 __device__ __forceinline__ double band_row_sum(const S *band, int tb, int64_t I, int64_t K0,
-                                               int64_t K1, const double (*ring)[T]) {
+                                               int64_t K1, const double (*ring)[T],
+                                               double acc = 0.0) {
   const int i = threadIdx.x >> 4, q = threadIdx.x & 15;
-  double acc = 0.0;
   for (int64_t Kb = K0; Kb < K1; Kb += B) {
     typename Vec4<S>::type m[B];
 #pragma unroll
@@ -719,13 +719,34 @@ __device__ __forceinline__ void fwd_sweep_mw_body(int w, int W, int64_t nt, int 
       ml = *reinterpret_cast<const typename Vec4<S>::type *>(tile_ptr(band, tb, I, I - 1) + i * T +
                                                              4 * q);
     double acc = 0.0;
-    if (I - 1 > K0) {
-      stage_rows(y, have + 1 > K0 ? have + 1 : K0, I - 2, ring, &bad);
-      if (I - 2 > have) have = I - 2;
+    // rows up to I - W are published (this workgroup's previous row and all
+    // it staged before): their tiles are summed without waiting; rows
+    // I - W + 1 .. I - 2 follow one at a time, each as it appears, in the
+    // same increasing order, so the sum does not depend on W
+    const int64_t Ka = I - W < I - 2 ? I - W : I - 2;
+    if (Ka >= K0) {
+      stage_rows(y, have + 1 > K0 ? have + 1 : K0, Ka, ring, &bad);
+      if (Ka > have) have = Ka;
       __syncthreads();
       if (bad) break;
-      acc = band_row_sum<S, sizeof(S) == 4 ? 12 : 8>(band, tb, I, K0, I - 1, ring);
+      acc = band_row_sum<S, sizeof(S) == 4 ? 12 : 8>(band, tb, I, K0, Ka + 1, ring);
     }
+    for (int64_t r = Ka + 1 > K0 ? Ka + 1 : K0; r <= I - 2; ++r) {
+      const typename Vec4<S>::type m =
+          *reinterpret_cast<const typename Vec4<S>::type *>(tile_ptr(band, tb, I, r) + i * T + 4 * q);
+      if (r > have) {
+        stage_rows(y, r, r, ring, &bad);
+        have = r;
+      }
+      __syncthreads();
+      if (bad) break;
+      const double *x = ring[r % RING] + 4 * q;
+      acc = fma((double)m.x, x[0], acc);
+      acc = fma((double)m.y, x[1], acc);
+      acc = fma((double)m.z, x[2], acc);
+      acc = fma((double)m.w, x[3], acc);
+    }
+    if (bad) break;
     if (I >= 1) {
       stage_rows(y, have + 1 > I - 1 ? have + 1 : I - 1, I - 1, ring, &bad);
       if (I - 1 > have) have = I - 1;
