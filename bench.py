@@ -420,7 +420,20 @@ def main():
     ap.add_argument("--nmin", type=int, default=16)
     ap.add_argument("--nmax", type=int, default=96)
     ap.add_argument("--sample", type=int, default=8)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="one 256 MiB device-to-device copy and exit (calibrates HBM "
+                         "counters: FETCH_SIZE / WRITE_SIZE of a copy of known size)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        n = 256 << 20
+        src = torch.ones(n // 4, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n, "ok": bool(dst[-1].item() == 1.0)}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(independent patches shard across GPUs with no communication; see DESIGN.md)",
@@ -518,8 +531,18 @@ def main():
     flops_diag = 2.0 / 3.0 * TILE ** 3
     d_us = float(np.mean(diag_us)) if diag_us else float("nan")
     achieved = flops_diag / (d_us * 1e-6) / 1e12
+    # HBM bytes per k_diag launch, from two rocprofv3 --pmc passes over
+    # `bench.py --steps 1 --warmup 0 --no-cpu-baseline` (FETCH_SIZE, then
+    # WRITE_SIZE, KiB per dispatch averaged over its k_diag dispatches), each
+    # scaled by the same counter's reading for the 256 MiB copy of
+    # `bench.py --copy-probe` (FETCH_SIZE 131088.375 KiB, WRITE_SIZE 262144 KiB
+    # for 262144 KiB moved); profiles/k_diag_hbm_traffic_pmc.txt holds them.
+    diag_fetch_kib, diag_write_kib = 76.5625, 32.25
+    traffic = (diag_fetch_kib * (262144.0 / 131088.375) + diag_write_kib * 1.0) * 1024.0
     res["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                       "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                       "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                       "traffic": traffic, "traffic_unit": "bytes per launch",
+                       "algorithmic_bytes": 2.0 * TILE * TILE * 8 + TILE * 4,
                        "kernel": "k_diag (64x64 tile LU with partial pivoting, one wavefront)",
                        "flops_per_launch": flops_diag, "avg_us": d_us,
                        "launches_per_factorization": nt,
