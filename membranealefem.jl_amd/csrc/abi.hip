@@ -98,6 +98,11 @@ int from_hip(Ctx &c, const HipError &h) {
 // The supported DOF layouts are those of get_dofs (src/input/Bc.jl:414-431).
 bool layout_ok(const mafem_mesh_desc *m, int motion, int &ndf) {
   auto eq3 = [](const int32_t *a, int x, int y, int z) { return a[0] == x && a[1] == y && a[2] == z; };
+  if (motion == STATIC) {  // flat 2-D scenarios: vx, vy, λ (Bc.jl:58-157, 286-316)
+    ndf = 3;
+    return eq3(m->v_order, 1, 2, 0) && eq3(m->m_order, 0, 0, 0) && m->lam_order == 3 &&
+           m->p_order == 0;
+  }
   if (!eq3(m->v_order, 1, 2, 3)) return false;
   if (motion == LAG) {
     ndf = 4;
@@ -121,7 +126,7 @@ void build(Ctx &c, const mafem_mesh_desc *m, const mafem_params *p) {
     throw ArgError{"inconsistent flat-patch mesh sizes", MAFEM_EINVAL};
   int ndf = 0;
   if (!layout_ok(m, p->motion, ndf))
-    throw ArgError{"unsupported motion / DOF layout (supported: LAG, EUL, ALEV, ALEVB with "
+    throw ArgError{"unsupported motion / DOF layout (supported: LAG, EUL, ALEV, ALEVB, STATIC 2-D with "
                    "the get_dofs ordering)",
                    MAFEM_EUNSUPPORTED};
   if (m->ndf != ndf) throw ArgError{"ndf does not match the DOF layout", MAFEM_EINVAL};
@@ -857,6 +862,7 @@ int mafem_pull_force(mafem_ctx *ctx, double *x_pull, double *f_pull) {
   ABI_GUARD(ctx, {
     if (!x_pull || !f_pull) return fail(c, "null output", MAFEM_EINVAL);
     if (!c.pf_err.empty()) return fail(c, c.pf_err, MAFEM_EINVAL);
+    if (c.motion == STATIC) return fail(c, "f_pull needs 3-D velocity", MAFEM_EINVAL);
     MAFEM_HIP(hipSetDevice(c.dev));
     ensure_pull(c);
     launch_pull_force(c, c.stream);

@@ -38,11 +38,14 @@ __device__ __forceinline__ D dsqrt(D a) {
 // DOF layout of mesh.dofs for each motion (get_dofs, src/input/Bc.jl:414-431)
 template <int MO>
 struct Lay {
-  static constexpr int NDF = (MO == LAG || MO == STATIC) ? 4 : (MO == EUL ? 7 : 8);
+  // STATIC: the flat 2-D scenarios (F_CAVI, F_COUE, F_POIS; Bc.jl:58-157,
+  // 286-316) with unknowns vx, vy, λ and no vz
+  static constexpr int NDF = MO == STATIC ? 3 : (MO == LAG ? 4 : (MO == EUL ? 7 : 8));
   static constexpr int NE = 9 * NDF;
+  static constexpr int NV = MO == STATIC ? 2 : 3;  // velocity components with unknowns
   static constexpr bool HAS_VM = (MO == EUL || MO == ALEV || MO == ALEVB);
   static constexpr bool HAS_PM = (MO == ALEV || MO == ALEVB);
-  static constexpr int LAMI = (MO == LAG || MO == STATIC) ? 3 : 6;
+  static constexpr int LAMI = MO == STATIC ? 2 : (MO == LAG ? 3 : 6);
   static constexpr int PMI = 7;
   // first column of the motion DOFs (get_m_motion_order, Mesh.jl:529-542)
   static constexpr int MOT0 = (MO == LAG) ? 0 : (MO == STATIC ? -1 : 3);
@@ -204,14 +207,16 @@ __device__ __forceinline__ void area_body(
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       double s = 0, s1 = 0, s2 = 0;
+      if (i < L::NV) {  // an absent vz is zero (get_dof_cps, GeoDynStress.jl:190-205)
 #pragma unroll
-      for (int a = 0; a < 9; ++a) {
-        const double c = C[i * 9 + a];
-        s = fma(c, bN[a], s);
-        s1 = fma(c, bN1[a], s1);
-        s2 = fma(c, bN2[a], s2);
+        for (int a = 0; a < 9; ++a) {
+          const double c = C[i * 9 + a];
+          s = fma(c, bN[a], s);
+          s1 = fma(c, bN1[a], s1);
+          s2 = fma(c, bN2[a], s2);
+        }
       }
-      const bool sd = (d0 == i);
+      const bool sd = (d0 == i) && i < L::NV;
       v[i] = mk(s, sd ? tN : 0.0);
       dv0[i] = mk(s1, sd ? tN1 : 0.0);
       dv1[i] = mk(s2, sd ? tN2 : 0.0);
@@ -328,7 +333,7 @@ __device__ __forceinline__ void area_body(
       D nn2 = mk(b12[a], 0.0) - (n1 * G0[2] + n2 * G1[2]);
       D m = nn0 * Mv[0] + nn1 * Mv[1] + nn2 * Mv[2];
 #pragma unroll
-      for (int i = 0; i < 3; ++i) {
+      for (int i = 0; i < L::NV; ++i) {
         D rv = A1[i] * s1 + A2[i] * s2 + Nn[i] * m;
         if (ph.pn != 0.0) rv = rv - (ph.pn * na) * Nn[i];
         ADD(a * NDF + i, rv);
@@ -465,7 +470,7 @@ __device__ __forceinline__ void edge_body(
 #pragma unroll
       for (int a = 0; a < 9; ++a)
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
+        for (int i = 0; i < L::NV; ++i) {
           const D val = -(bN[a] * fv[i]);
           acc[a * 3 + i] += val.t;
           racc += (a * NDF + i == col) ? val.v : 0.0;
@@ -478,7 +483,7 @@ __device__ __forceinline__ void edge_body(
       for (int a = 0; a < 9; ++a) {
         const D dn = bN1[a] * nu0 + bN2[a] * nu1;
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
+        for (int i = 0; i < L::NV; ++i) {
           const D val = -((g.n[i] * dn) * s);
           acc[a * 3 + i] += val.t;
           racc += (a * NDF + i == col) ? val.v : 0.0;
@@ -491,7 +496,7 @@ __device__ __forceinline__ void edge_body(
 #pragma unroll
     for (int row = 0; row < NE; ++row) {
       const int a = row / NDF, d = row % NDF;
-      K[row * NE + col] = (d < 3) ? acc[a * 3 + d] : 0.0;
+      K[row * NE + col] = (d < L::NV) ? acc[a * 3 + d] : 0.0;
     }
     rel[k * NE + col] = racc;
   }
@@ -744,6 +749,7 @@ static hipError_t launch_area(Ctx &c, double dt) {
                      c.d_uel, c.d_active, c.d_xms, c.d_cps, c.d_ufns, c.d_dbt, c.phys, dt, \
                      c.d_kel, c.d_rel)
   switch (c.motion) {
+    case STATIC: AREA(STATIC); break;
     case LAG: AREA(LAG); break;
     case EUL: AREA(EUL); break;
     case ALEV: AREA(ALEV); break;
@@ -764,6 +770,7 @@ static hipError_t launch_edge(Ctx &c, double time, double dt) {
   hipLaunchKernelGGL(k_edge<MO>, grid, dim3(256), 0, c.stream, c.n_edges, c.numnp, c.d_edges, \
                      c.d_ix, c.d_active, c.d_xms, c.d_cps, c.d_bufns, c.phys, time, dt, kel, rel)
   switch (c.motion) {
+    case STATIC: EDGE(STATIC); break;
     case LAG: EDGE(LAG); break;
     case EUL: EDGE(EUL); break;
     case ALEV: EDGE(ALEV); break;
@@ -845,6 +852,7 @@ void b_assemble(BatchPlan &B) {
     hipLaunchKernelGGL(kb_edge<MO>, dim3(se.nblk), dim3(256), 0, B.stream, se, B.d_pd, B.f.on, \
                        B.f.time, B.f.dt)
     switch (B.motions[k]) {
+      case STATIC: BAREA(STATIC); break;
       case LAG: BAREA(LAG); break;
       case EUL: BAREA(EUL); break;
       case ALEV: BAREA(ALEV); break;

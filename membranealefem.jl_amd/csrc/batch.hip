@@ -687,8 +687,10 @@ int mafem_batch_pull_force(mafem_batch *bp, double *out) {
     }
     const int P = (int)b.members.size();
     for (int p = 0; p < P; ++p)
-      if (!b.members[p]->c.pf_err.empty()) {
-        b.err = "member " + std::to_string(p) + ": " + b.members[p]->c.pf_err;
+      if (!b.members[p]->c.pf_err.empty() || b.members[p]->c.motion == STATIC) {
+        b.err = "member " + std::to_string(p) + ": " +
+                (b.members[p]->c.pf_err.empty() ? std::string("f_pull needs 3-D velocity")
+                                                : b.members[p]->c.pf_err);
         return MAFEM_EINVAL;
       }
     MAFEM_HIP(hipSetDevice(b.dev));
