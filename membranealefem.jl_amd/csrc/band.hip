@@ -118,20 +118,17 @@ __device__ __forceinline__ float lane_bcast(float x, int l) {
 // candidate row takes the rank-16 update A22 -= L21 U12 from broadcast LDS
 // reads.  Same arithmetic as the unblocked elimination, with the per-step
 // chain shortened from 64 - j to at most 15 columns.
+// The tile is already in st (st[i][0..63] = row i); only wave 0 of the
+// block works, other waves (when the block is larger) take part in the
+// barriers only.  The factored tile goes to A, the pivots to piv[K].
 template <class S>
-__device__ __forceinline__ void diag_body(int64_t K, int tb, S *__restrict__ band,
-                                          int32_t *__restrict__ piv) {
-  __shared__ S st[T][T + 1];
-  __shared__ S u12[16][T];
-  __shared__ S l11[16][16];
-  const int lane = threadIdx.x;
-  S *A = tile_ptr(band, tb, K, K);
-#pragma unroll 8
-  for (int i = 0; i < T; ++i) st[i][lane] = A[i * T + lane];
-  __syncthreads();
+__device__ __forceinline__ void diag_core(int64_t K, S *__restrict__ A, int32_t *__restrict__ piv,
+                                          S (*st)[T + 1], S (*u12)[T], S (*l11)[16]) {
+  const bool on = threadIdx.x < 64;
+  const int lane = (int)(threadIdx.x & 63);
   S r[T];
 #pragma unroll
-  for (int k = 0; k < T; ++k) r[k] = st[lane][k];
+  for (int k = 0; k < T; ++k) r[k] = on ? st[lane][k] : S(0);
   int pos = -1;
 #pragma unroll
   for (int pb = 0; pb < 4; ++pb) {
@@ -150,7 +147,7 @@ __device__ __forceinline__ void diag_body(int64_t K, int tb, S *__restrict__ ban
     }
     if (pb == 3) break;
     const bool mine = pos >= c0 && pos < c0 + 16;  // pivot row of this panel
-    if (mine) {
+    if (on && mine) {
       const int i = pos - c0;
 #pragma unroll
       for (int k = c0 + 16; k < T; ++k) u12[i][k] = r[k];
@@ -158,7 +155,7 @@ __device__ __forceinline__ void diag_body(int64_t K, int tb, S *__restrict__ ban
       for (int jj = 0; jj < 16; ++jj) l11[i][jj] = (jj < i) ? r[c0 + jj] : S(0);
     }
     __syncthreads();
-    if (lane >= c0 + 16) {  // U12 = L11^-1 A12, one trailing column per lane
+    if (on && lane >= c0 + 16) {  // U12 = L11^-1 A12, one trailing column per lane
       S u[16];
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
@@ -171,7 +168,7 @@ __device__ __forceinline__ void diag_body(int64_t K, int tb, S *__restrict__ ban
       for (int i = 0; i < 16; ++i) u12[i][lane] = u[i];
     }
     __syncthreads();
-    if (pos < 0) {  // A22 -= L21 U12
+    if (on && pos < 0) {  // A22 -= L21 U12
 #pragma unroll
       for (int k = c0 + 16; k < T; ++k) {
         S acc = r[k];
@@ -179,19 +176,37 @@ __device__ __forceinline__ void diag_body(int64_t K, int tb, S *__restrict__ ban
         for (int jj = 0; jj < 16; ++jj) acc = fma(-r[c0 + jj], u12[jj][k], acc);
         r[k] = acc;
       }
-    } else if (mine) {
+    } else if (on && mine) {
 #pragma unroll
       for (int k = c0 + 16; k < T; ++k) r[k] = u12[pos - c0][k];
     }
     __syncthreads();
   }
   __syncthreads();
+  if (on) {
 #pragma unroll
-  for (int k = 0; k < T; ++k) st[pos][k] = r[k];
-  piv[K * T + pos] = lane;
+    for (int k = 0; k < T; ++k) st[pos][k] = r[k];
+    piv[K * T + pos] = lane;
+  }
   __syncthreads();
+  if (on) {
 #pragma unroll 8
-  for (int i = 0; i < T; ++i) A[i * T + lane] = st[i][lane];
+    for (int i = 0; i < T; ++i) A[i * T + lane] = st[i][lane];
+  }
+}
+
+template <class S>
+__device__ __forceinline__ void diag_body(int64_t K, int tb, S *__restrict__ band,
+                                          int32_t *__restrict__ piv) {
+  __shared__ S st[T][T + 1];
+  __shared__ S u12[16][T];
+  __shared__ S l11[16][16];
+  const int lane = threadIdx.x;
+  S *A = tile_ptr(band, tb, K, K);
+#pragma unroll 8
+  for (int i = 0; i < T; ++i) st[i][lane] = A[i * T + lane];
+  __syncthreads();
+  diag_core<S>(K, A, piv, st, u12, l11);
 }
 
 template <class S>
@@ -344,14 +359,15 @@ __device__ __forceinline__ int mm_row(int j) {
 template <class S>
 __device__ __forceinline__ int mm_col(int bj) { return 16 * bj + (int)(threadIdx.x & 15); }
 
+constexpr int MM_LDS = 2 * T * (T + 1);  // entries of the LDS buffer of tile_mm
+
 template <class S>
 __device__ __forceinline__ void tile_mm(const S *__restrict__ a, const S *__restrict__ b,
-                                        S c[4][4]) {
+                                        S c[4][4], S *lds) {
   using V = typename Mfma<S>::V;
   constexpr int P = T + 1;
   constexpr int W = 16 / sizeof(S);  // entries per 16-byte load
-  __shared__ S sa[T * P];
-  __shared__ S sb[T * P];
+  S *sa = lds, *sb = lds + T * P;
   struct alignas(16) Vec { S v[W]; };
   Vec va[T * T / W / 256], vb[T * T / W / 256];
 #pragma unroll
@@ -389,8 +405,11 @@ __device__ __forceinline__ void tile_mm(const S *__restrict__ a, const S *__rest
     for (int j = 0; j < 4; ++j) c[bj][j] = acc[bj][j];
 }
 
+// to_lds: the updated tile goes to lds as rows of T + 1 entries (for the
+// diagonal LU that follows in the same block) instead of to the band.
 template <class S>
-__device__ __forceinline__ void update_body(int64_t bid, int64_t K, int64_t nt, int tb, S *__restrict__ band) {
+__device__ __forceinline__ void update_body(int64_t bid, int64_t K, int64_t nt, int tb, S *__restrict__ band,
+                                            S *lds, bool to_lds = false) {
   const int64_t I = K + 1 + bid / tb, J = K + 1 + bid % tb;
   if (I >= nt || J >= nt) return;
   const S *Lt = tile_ptr(band, tb, I, K);
@@ -402,7 +421,15 @@ __device__ __forceinline__ void update_body(int64_t bid, int64_t K, int64_t nt, 
 #pragma unroll
     for (int j = 0; j < 4; ++j) old[bj][j] = A[mm_row<S>(j) * T + mm_col<S>(bj)];
   S c[4][4];
-  tile_mm<S>(Lt, Ut, c);
+  tile_mm<S>(Lt, Ut, c, lds);
+  if (to_lds) {
+    __syncthreads();  // every wavefront is done with the staged operands
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) lds[mm_row<S>(j) * (T + 1) + mm_col<S>(bj)] = old[bj][j] - c[bj][j];
+    return;
+  }
 #pragma unroll
   for (int bj = 0; bj < 4; ++bj)
 #pragma unroll
@@ -411,7 +438,25 @@ __device__ __forceinline__ void update_body(int64_t bid, int64_t K, int64_t nt, 
 
 template <class S>
 __global__ __launch_bounds__(256) void k_update(int64_t K, int64_t nt, int tb, S *__restrict__ band) {
-  update_body<S>((int64_t)blockIdx.x, K, nt, tb, band);
+  __shared__ __align__(16) S lds[MM_LDS];
+  update_body<S>((int64_t)blockIdx.x, K, nt, tb, band, lds);
+}
+
+// Trailing update of step K with the next diagonal LU fused in: block 0
+// updates tile (K+1, K+1) into LDS, then its first wavefront factors it
+// while the other blocks are still updating (requires K + 1 < nt).
+template <class S>
+__global__ __launch_bounds__(256) void k_update_diag(int64_t K, int64_t nt, int tb,
+                                                     S *__restrict__ band, int32_t *__restrict__ piv) {
+  __shared__ __align__(16) S lds[MM_LDS];
+  const bool fused = blockIdx.x == 0;
+  update_body<S>((int64_t)blockIdx.x, K, nt, tb, band, lds, fused);
+  if (!fused) return;
+  __syncthreads();
+  S(*st)[T + 1] = reinterpret_cast<S(*)[T + 1]>(lds);
+  S(*u12)[T] = reinterpret_cast<S(*)[T]>(lds + T * (T + 1));
+  S(*l11)[16] = reinterpret_cast<S(*)[16]>(lds + T * (T + 1) + 16 * T);
+  diag_core<S>(K + 1, tile_ptr(band, tb, K + 1, K + 1), piv, st, u12, l11);
 }
 
 // ------------------------------------------------- post-factorization pass
@@ -464,7 +509,7 @@ __global__ __launch_bounds__(64) void k_tile_inv(int tb, const S *__restrict__ b
 template <class S>
 __device__ __forceinline__ void fold_body(int64_t bid, int64_t nt, int tb, S *__restrict__ band,
                                               const S *__restrict__ linv,
-                                              const S *__restrict__ uinv) {
+                                              const S *__restrict__ uinv, S *lds) {
   const int64_t I = bid / (2 * tb + 1);
   const int slot = bid % (2 * tb + 1);
   const int64_t J = I - tb + slot;
@@ -472,7 +517,7 @@ __device__ __forceinline__ void fold_body(int64_t bid, int64_t nt, int tb, S *__
   const S *Inv = (slot < tb ? linv : uinv) + I * (int64_t)T * T;
   S *A = tile_ptr(band, tb, I, J);
   S c[4][4];
-  tile_mm<S>(Inv, A, c);  // reads all of A before its barrier
+  tile_mm<S>(Inv, A, c, lds);  // reads all of A before its barrier
 #pragma unroll
   for (int bj = 0; bj < 4; ++bj)
 #pragma unroll
@@ -483,7 +528,8 @@ template <class S>
 __global__ __launch_bounds__(256) void k_fold(int64_t nt, int tb, S *__restrict__ band,
                                               const S *__restrict__ linv,
                                               const S *__restrict__ uinv) {
-  fold_body<S>((int64_t)blockIdx.x, nt, tb, band, linv, uinv);
+  __shared__ __align__(16) S lds[MM_LDS];
+  fold_body<S>((int64_t)blockIdx.x, nt, tb, band, linv, uinv, lds);
 }
 
 // ------------------------------------------------------------ triangular solves
@@ -939,7 +985,8 @@ __global__ __launch_bounds__(256) void kb_update(StepMap sm, int64_t K, const in
   int local, tb;
   const int p = step_locate<2>(sm, order, local, tb);
   if (!fac_on(on, fac, p) || pd[p].prec != prec_of<S>()) return;
-  update_body<S>(local, K, pd[p].nt, tb, (S *)pd[p].band);
+  __shared__ __align__(16) S lds[MM_LDS];
+  update_body<S>(local, K, pd[p].nt, tb, (S *)pd[p].band, lds);
 }
 
 template <class S>
@@ -959,8 +1006,9 @@ __global__ __launch_bounds__(256) void kb_fold(Seg sg, const PatchDev *__restric
   const int p = sg.map[blockIdx.x];
   if (!fac_on(on, fac, p) || pd[p].prec != prec_of<S>()) return;
   const PatchDev &d = pd[p];
+  __shared__ __align__(16) S lds[MM_LDS];
   fold_body<S>(blockIdx.x - sg.blk0[p], d.nt, d.tb, (S *)d.band, (const S *)d.linv,
-               (const S *)d.uinv);
+               (const S *)d.uinv, lds);
 }
 
 // which = 0: t2 = L^-1 P w; which = 1: t = U^-1 t2 (per tile row)
@@ -1059,17 +1107,31 @@ static void fill_t(Ctx &c, S *band) {
 
 template <class S>
 static void factor_t(Ctx &c, S *band, S *li, S *ui) {
+  // the diagonal LU of step K + 1 runs inside step K's trailing update
+  // (k_update_diag), except for one representative step per factorization
+  // whose diagonal LU is launched (and timed) on its own
+  const int64_t timed_K = c.nt / 2;
+  bool have_diag = false;
   for (int64_t K = 0; K < c.nt; ++K) {
-    const bool timed = (K == c.nt / 2);  // one representative launch per factorization
-    if (timed) MAFEM_HIP(hipEventRecord(c.ev[10], c.stream));
-    hipLaunchKernelGGL(k_diag<S>, dim3(1), dim3(64), 0, c.stream, K, c.tb, band, c.d_piv);
-    if (timed) {
-      MAFEM_HIP(hipEventRecord(c.ev[11], c.stream));
-      c.diag_timed = true;
+    if (!have_diag) {
+      const bool timed = (K == timed_K);
+      if (timed) MAFEM_HIP(hipEventRecord(c.ev[10], c.stream));
+      hipLaunchKernelGGL(k_diag<S>, dim3(1), dim3(64), 0, c.stream, K, c.tb, band, c.d_piv);
+      if (timed) {
+        MAFEM_HIP(hipEventRecord(c.ev[11], c.stream));
+        c.diag_timed = true;
+      }
     }
     hipLaunchKernelGGL(k_panel<S>, dim3(3 * c.tb), dim3(64), 0, c.stream, K, c.nt, c.tb, band,
                        c.d_piv);
-    hipLaunchKernelGGL(k_update<S>, dim3(c.tb * c.tb), dim3(256), 0, c.stream, K, c.nt, c.tb, band);
+    const bool fuse = K + 1 < c.nt && K + 1 != timed_K;
+    if (fuse)
+      hipLaunchKernelGGL(k_update_diag<S>, dim3(c.tb * c.tb), dim3(256), 0, c.stream, K, c.nt,
+                         c.tb, band, c.d_piv);
+    else
+      hipLaunchKernelGGL(k_update<S>, dim3(c.tb * c.tb), dim3(256), 0, c.stream, K, c.nt, c.tb,
+                         band);
+    have_diag = fuse;
   }
   hipLaunchKernelGGL(k_tile_inv<S>, dim3((unsigned)c.nt), dim3(64), 0, c.stream, c.tb, band, li, ui);
   hipLaunchKernelGGL(k_fold<S>, dim3((unsigned)(c.nt * (2 * c.tb + 1))), dim3(256), 0, c.stream,
