@@ -51,6 +51,11 @@ def pull_speed(k, total=200, final=2.0):
     return final * (k + 1) / total
 
 
+# k_update_diag HBM counters per dispatch (KiB, mean over the 2176 launches of
+# one time step of the 64x64 ALEVB bench patch); see the roofline below
+STEP_FETCH_KIB, STEP_WRITE_KIB = 8914.95051125919, 9064.25
+
+
 def band_update_flops(dev):
     """Algorithmic FLOPs of the band-LU trailing updates of one factorization:
     2 T^3 per tile update, over the (tb x tb) window clipped at the end."""
@@ -477,7 +482,7 @@ def main():
     agg = {"assemble_us": 0.0, "area_kernel_us": 0.0, "factor_us": 0.0, "solve_us": 0.0,
            "update_us": 0.0, "n_assemble": 0, "n_band_solve": 0, "n_factor": 0,
            "gmres_iters": 0}
-    diag_us = []
+    step_us = []
     relres = []
     t_start = time.perf_counter()
     for k in range(args.warmup, args.warmup + args.steps):
@@ -488,8 +493,8 @@ def main():
         relres.append(st["last_relres"])
         for key in agg:
             agg[key] += st[key]
-        if st["n_factor"] > 0 and st["diag_kernel_us"] > 0:
-            diag_us.append(st["diag_kernel_us"])
+        if st["n_factor"] > 0 and st["step_kernel_us"] > 0:
+            step_us.append(st["step_kernel_us"])
     t_total = time.perf_counter() - t_start
 
     numel = mesh.numel
@@ -525,26 +530,34 @@ def main():
         "band_solves": agg["n_band_solve"],
         "rel_err_vs_fp64": 0.0,
     }
-    # dominant kernel of the direct FP64 path: the diagonal-tile LU (k_diag),
-    # launched once per tile row of every factorization.  Algorithmic work per
-    # launch: LU with partial pivoting of a 64x64 tile, 2/3 * 64^3 FLOP.
-    flops_diag = 2.0 / 3.0 * TILE ** 3
-    d_us = float(np.mean(diag_us)) if diag_us else float("nan")
-    achieved = flops_diag / (d_us * 1e-6) / 1e12
-    # HBM bytes per k_diag launch, from two rocprofv3 --pmc passes over
+    # dominant kernel of the direct FP64 path: k_update_diag, one launch per
+    # tile row of every factorization: the trailing update of step K (tb x tb
+    # tile products A_IJ -= L_IK U_KJ on the matrix cores, 2 * 64^3 FLOP each)
+    # with the 64x64 diagonal LU of step K + 1 (2/3 * 64^3 FLOP) fused in;
+    # the timed launch is the middle step, where all tb^2 tiles are in range.
+    _, tb = dev_band_shape(dev)
+    kmid = nt // 2
+    n_tiles = min(tb, nt - 1 - kmid) ** 2
+    flops_step = n_tiles * 2.0 * TILE ** 3 + 2.0 / 3.0 * TILE ** 3
+    s_us = float(np.mean(step_us)) if step_us else float("nan")
+    achieved = flops_step / (s_us * 1e-6) / 1e12
+    # HBM bytes per k_update_diag launch, from two rocprofv3 --pmc passes over
     # `bench.py --steps 1 --warmup 0 --no-cpu-baseline` (FETCH_SIZE, then
-    # WRITE_SIZE, KiB per dispatch averaged over its k_diag dispatches), each
-    # scaled by the same counter's reading for the 256 MiB copy of
+    # WRITE_SIZE, KiB per dispatch averaged over its k_update_diag dispatches),
+    # each scaled by the same counter's reading for the 256 MiB copy of
     # `bench.py --copy-probe` (FETCH_SIZE 131088.375 KiB, WRITE_SIZE 262144 KiB
-    # for 262144 KiB moved); profiles/k_diag_hbm_traffic_pmc.txt holds them.
-    diag_fetch_kib, diag_write_kib = 76.5625, 32.25
-    traffic = (diag_fetch_kib * (262144.0 / 131088.375) + diag_write_kib * 1.0) * 1024.0
+    # for 262144 KiB moved); profiles/k_update_diag_hbm_traffic_pmc.txt holds them.
+    traffic = None
+    if STEP_FETCH_KIB is not None:
+        traffic = (STEP_FETCH_KIB * (262144.0 / 131088.375) + STEP_WRITE_KIB * 1.0) * 1024.0
+    s_bytes = TILE * TILE * 8
     res["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                        "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                        "traffic": traffic, "traffic_unit": "bytes per launch",
-                       "algorithmic_bytes": 2.0 * TILE * TILE * 8 + TILE * 4,
-                       "kernel": "k_diag (64x64 tile LU with partial pivoting, one wavefront)",
-                       "flops_per_launch": flops_diag, "avg_us": d_us,
+                       "algorithmic_bytes": (2 * n_tiles + 2 * tb) * s_bytes,
+                       "kernel": "k_update_diag (trailing update of one factorization step on "
+                                 "the matrix cores, next 64x64 diagonal LU fused in)",
+                       "flops_per_launch": flops_step, "avg_us": s_us,
                        "launches_per_factorization": nt,
                        "factorization_tflops": flops_fact / (agg["factor_us"] * 1e-6 /
                                                              max(1, agg["n_factor"])) / 1e12

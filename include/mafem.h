@@ -111,8 +111,9 @@ typedef struct {
   int32_t newton_iters, refine_sweeps;
   int32_t n_assemble, n_band_solve;  /* launches of the assembly / triangular-solve sweeps */
   int32_t n_factor, gmres_iters;     /* band factorizations / Krylov iterations */
-  double diag_kernel_us;           /* mean duration of the diagonal-tile LU kernel
-                                      (its middle launch in each factorization) */
+  double step_kernel_us;           /* mean duration of one factorization-step kernel
+                                      (trailing update with the next diagonal-tile LU
+                                      fused; the middle step of each factorization) */
   double last_relres;              /* |K du + r| / |r| of the last solve */
   int32_t sweep_groups;            /* workgroups of each triangular sweep (autotuned) */
   int32_t reserved;

@@ -808,8 +808,8 @@ int mafem_time_step(mafem_ctx *ctx, double time, double dt, int32_t max_iter, in
     double eps = INFINITY;
     int it = 0;
     const int32_t nbs0 = c.n_band_solve, nf0 = c.n_factor, ng0 = c.n_gmres;
-    int n_diag = 0;
-    c.diag_timed = false;
+    int n_step = 0;
+    c.step_timed = false;
     MAFEM_HIP(hipEventRecord(c.ev[0], c.stream));
     while (it < max_iter) {
       MAFEM_HIP(hipEventRecord(c.ev[1], c.stream));
@@ -827,10 +827,10 @@ int mafem_time_step(mafem_ctx *ctx, double time, double dt, int32_t max_iter, in
       st.solve_us += 1e3 * elapsed(c, 3, 4);
       st.update_us += 1e3 * elapsed(c, 4, 5);
       st.refine_sweeps += c.stats.refine_sweeps;
-      if (c.diag_timed) {
-        st.diag_kernel_us += 1e3 * elapsed(c, 10, 11);
-        ++n_diag;
-        c.diag_timed = false;
+      if (c.step_timed) {
+        st.step_kernel_us += 1e3 * elapsed(c, 10, 11);
+        ++n_step;
+        c.step_timed = false;
       }
       st.last_relres = c.stats.last_relres;
       if (eps_hist) eps_hist[it] = eps;
@@ -846,7 +846,7 @@ int mafem_time_step(mafem_ctx *ctx, double time, double dt, int32_t max_iter, in
     st.n_band_solve = c.n_band_solve - nbs0;
     st.n_factor = c.n_factor - nf0;
     st.gmres_iters = c.n_gmres - ng0;
-    if (n_diag > 0) st.diag_kernel_us /= n_diag;
+    if (n_step > 0) st.step_kernel_us /= n_step;
     st.sweep_groups = c.sweep_groups;
     c.stats = st;
     if (iters) *iters = it;

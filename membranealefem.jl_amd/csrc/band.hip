@@ -1108,30 +1108,21 @@ static void fill_t(Ctx &c, S *band) {
 template <class S>
 static void factor_t(Ctx &c, S *band, S *li, S *ui) {
   // the diagonal LU of step K + 1 runs inside step K's trailing update
-  // (k_update_diag), except for one representative step per factorization
-  // whose diagonal LU is launched (and timed) on its own
+  // (k_update_diag); one launch in the middle of the chain is timed
   const int64_t timed_K = c.nt / 2;
-  bool have_diag = false;
+  hipLaunchKernelGGL(k_diag<S>, dim3(1), dim3(64), 0, c.stream, (int64_t)0, c.tb, band, c.d_piv);
   for (int64_t K = 0; K < c.nt; ++K) {
-    if (!have_diag) {
-      const bool timed = (K == timed_K);
-      if (timed) MAFEM_HIP(hipEventRecord(c.ev[10], c.stream));
-      hipLaunchKernelGGL(k_diag<S>, dim3(1), dim3(64), 0, c.stream, K, c.tb, band, c.d_piv);
-      if (timed) {
-        MAFEM_HIP(hipEventRecord(c.ev[11], c.stream));
-        c.diag_timed = true;
-      }
-    }
     hipLaunchKernelGGL(k_panel<S>, dim3(3 * c.tb), dim3(64), 0, c.stream, K, c.nt, c.tb, band,
                        c.d_piv);
-    const bool fuse = K + 1 < c.nt && K + 1 != timed_K;
-    if (fuse)
-      hipLaunchKernelGGL(k_update_diag<S>, dim3(c.tb * c.tb), dim3(256), 0, c.stream, K, c.nt,
-                         c.tb, band, c.d_piv);
-    else
-      hipLaunchKernelGGL(k_update<S>, dim3(c.tb * c.tb), dim3(256), 0, c.stream, K, c.nt, c.tb,
-                         band);
-    have_diag = fuse;
+    if (K + 1 >= c.nt) break;  // no trailing tiles after the last step
+    const bool timed = (K == timed_K);
+    if (timed) MAFEM_HIP(hipEventRecord(c.ev[10], c.stream));
+    hipLaunchKernelGGL(k_update_diag<S>, dim3(c.tb * c.tb), dim3(256), 0, c.stream, K, c.nt, c.tb,
+                       band, c.d_piv);
+    if (timed) {
+      MAFEM_HIP(hipEventRecord(c.ev[11], c.stream));
+      c.step_timed = true;
+    }
   }
   hipLaunchKernelGGL(k_tile_inv<S>, dim3((unsigned)c.nt), dim3(64), 0, c.stream, c.tb, band, li, ui);
   hipLaunchKernelGGL(k_fold<S>, dim3((unsigned)(c.nt * (2 * c.tb + 1))), dim3(256), 0, c.stream,

@@ -109,7 +109,7 @@ struct Ctx {
   int band_prec = 0;
   bool have_factor = false;      // the band holds a factorization usable as preconditioner
   int32_t n_factor = 0;          // running count of factorizations
-  bool diag_timed = false;       // ev[10]/ev[11] bracket a k_diag launch not yet read
+  bool step_timed = false;       // ev[10]/ev[11] bracket a timed factorization-step launch not yet read
 
   // linear solver options and Krylov workspace
   mafem_solver_opts opts{};

@@ -53,7 +53,7 @@ class Stats(ctypes.Structure):
         ("newton_iters", ctypes.c_int32), ("refine_sweeps", ctypes.c_int32),
         ("n_assemble", ctypes.c_int32), ("n_band_solve", ctypes.c_int32),
         ("n_factor", ctypes.c_int32), ("gmres_iters", ctypes.c_int32),
-        ("diag_kernel_us", ctypes.c_double),
+        ("step_kernel_us", ctypes.c_double),
         ("last_relres", ctypes.c_double),
         ("sweep_groups", ctypes.c_int32), ("reserved", ctypes.c_int32),
     ]
