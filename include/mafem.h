@@ -116,7 +116,10 @@ typedef struct {
                                       fused; the middle step of each factorization) */
   double last_relres;              /* |K du + r| / |r| of the last solve */
   int32_t sweep_groups;            /* workgroups of each triangular sweep (autotuned) */
-  int32_t reserved;
+  int32_t pivot_perturbations;     /* diagonal-tile pivots below eps max|K| (numerically
+                                      zero) replaced by +-eps max|K|: static pivoting,
+                                      switched on for a context once a factorization
+                                      without it gave a failed solve */
 } mafem_stats;
 
 /* Builds the device context: uploads the tables, builds the block-sparse

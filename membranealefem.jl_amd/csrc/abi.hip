@@ -57,7 +57,7 @@ void free_all(Ctx &c) {
                   c.d_uinv, c.d_piv,  c.d_V,       c.d_Z,       c.d_h,      c.d_coef,
                   c.d_mdot_part, c.d_t2, c.d_presc_idx, c.d_presc_val, c.d_s1, c.d_s2,
                   c.d_sweep_err, c.d_pf_el, c.d_pf_slot, c.d_pf_rel, c.d_pf_out,
-                  c.d_obel, c.d_crnr, c.d_out};
+                  c.d_obel, c.d_crnr, c.d_out, c.d_amax};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c.h_red) (void)hipHostFree(c.h_red);
@@ -87,6 +87,14 @@ namespace {
 int fail(Ctx &c, const std::string &m, int code) {
   c.err = m;
   return code;
+}
+
+// running count of replaced pivots (blocking read of the device counter)
+int32_t read_pert(Ctx &c) {
+  unsigned long long v[2] = {0, 0};
+  MAFEM_HIP(hipMemcpyAsync(v, c.d_amax, sizeof(v), hipMemcpyDeviceToHost, c.stream));
+  MAFEM_HIP(hipStreamSynchronize(c.stream));
+  return (int32_t)v[1];
 }
 
 int from_hip(Ctx &c, const HipError &h) {
@@ -422,6 +430,8 @@ void build(Ctx &c, const mafem_mesh_desc *m, const mafem_params *p) {
   c.d_s1 = dalloc<double>(c, nv);
   c.d_s2 = dalloc<double>(c, nv);
   c.d_sweep_err = dalloc<int>(c, 1);
+  c.d_amax = dalloc<unsigned long long>(c, 2);
+  MAFEM_HIP(hipMemset(c.d_amax, 0, 2 * sizeof(unsigned long long)));
   MAFEM_HIP(hipMemset(c.d_sweep_err, 0, sizeof(int)));
 }
 
@@ -553,7 +563,7 @@ void tune_sweeps(Ctx &c, const double *d_b) {
 
 void refactor(Ctx &c) {
   band_fill(c, c.opts.factor_precision);
-  band_factor(c);
+  band_factor(c, c.perturb_pivots);
   c.have_factor = true;
 }
 
@@ -566,8 +576,16 @@ double linear_solve(Ctx &c, const double *d_b, double *d_x) {
     refactor(c);
     if (!c.sweep_tuned) tune_sweeps(c, d_b);
     MAFEM_HIP(hipEventRecord(c.ev[3], c.stream));
-    const double rel = solve_refined(c, d_b, d_x, o.refine_sweeps, o.rtol);
+    double rel = solve_refined(c, d_b, d_x, o.refine_sweeps, o.rtol);
     if (rel <= 1e3 * o.rtol) return rel;
+    if (!c.perturb_pivots && !(rel <= 1e-2)) {
+      // a diagonal tile met a (numerically) zero pivot: from now on factor
+      // with static pivoting and solve again
+      c.perturb_pivots = true;
+      refactor(c);
+      rel = solve_refined(c, d_b, d_x, o.refine_sweeps, o.rtol);
+      if (rel <= 1e3 * o.rtol) return rel;
+    }
     // refinement stalls when the tile-local pivoting grew the factors: finish
     // with FGMRES preconditioned by the same fresh factorization
     ensure_krylov(c);
@@ -808,6 +826,7 @@ int mafem_time_step(mafem_ctx *ctx, double time, double dt, int32_t max_iter, in
     double eps = INFINITY;
     int it = 0;
     const int32_t nbs0 = c.n_band_solve, nf0 = c.n_factor, ng0 = c.n_gmres;
+    const int32_t np0 = read_pert(c);
     int n_step = 0;
     c.step_timed = false;
     MAFEM_HIP(hipEventRecord(c.ev[0], c.stream));
@@ -848,6 +867,7 @@ int mafem_time_step(mafem_ctx *ctx, double time, double dt, int32_t max_iter, in
     st.gmres_iters = c.n_gmres - ng0;
     if (n_step > 0) st.step_kernel_us /= n_step;
     st.sweep_groups = c.sweep_groups;
+    st.pivot_perturbations = read_pert(c) - np0;
     c.stats = st;
     if (iters) *iters = it;
     if (!(eps < c.enr)) {

@@ -121,11 +121,24 @@ __device__ __forceinline__ float lane_bcast(float x, int l) {
 // The tile is already in st (st[i][0..63] = row i); only wave 0 of the
 // block works, other waves (when the block is larger) take part in the
 // barriers only.  The factored tile goes to A, the pivots to piv[K].
-template <class S>
+// amax (used when PERT): static pivoting.  amax[0] holds max |K| as the bits of a
+// non-negative double; a pivot of magnitude below thr = eps max |K| (eps of
+// the storage precision: numerically zero) is replaced by +-thr (the
+// factorization is then that of a nearby matrix, and the solve finishes with
+// FGMRES) and counted in amax[1].
+template <class S, bool PERT = false>
 __device__ __forceinline__ void diag_core(int64_t K, S *__restrict__ A, int32_t *__restrict__ piv,
-                                          S (*st)[T + 1], S (*u12)[T], S (*l11)[16]) {
+                                          S (*st)[T + 1], S (*u12)[T], S (*l11)[16],
+                                          unsigned long long *amax = nullptr) {
   const bool on = threadIdx.x < 64;
   const int lane = (int)(threadIdx.x & 63);
+  S thr = S(0);
+  if (PERT) {
+    const double rel = sizeof(S) == 8 ? 2.220446049250313e-16 : 1.1920928955078125e-7;
+    thr = (S)(rel * __longlong_as_double((long long)__hip_atomic_load(
+                           amax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
+  }
+  int npert = 0;
   S r[T];
 #pragma unroll
   for (int k = 0; k < T; ++k) r[k] = on ? st[lane][k] : S(0);
@@ -138,10 +151,17 @@ __device__ __forceinline__ void diag_core(int64_t K, S *__restrict__ A, int32_t 
       const int j = c0 + jj;
       const unsigned key = wave_max_u32(pivot_key(r[j], pos < 0, lane));
       const int p = 63 - (int)(key & 63u);
-      const S pj = lane_bcast(r[j], p);
+      S pj = lane_bcast(r[j], p);
+      bool tiny = false;
+      if (PERT) {  // wave-uniform
+        tiny = fabs(pj) < thr;
+        pj = tiny ? (pj < S(0) ? -thr : thr) : pj;
+        npert += tiny ? 1 : 0;
+      }
       if (lane == p) pos = j;
       const S l = (pos < 0) ? r[j] / pj : S(0);
       if (pos < 0) r[j] = l;
+      if (PERT && tiny && lane == p) r[j] = pj;
 #pragma unroll
       for (int k = j + 1; k < c0 + 16; ++k) r[k] = fma(-l, lane_bcast(r[k], p), r[k]);
     }
@@ -187,6 +207,7 @@ __device__ __forceinline__ void diag_core(int64_t K, S *__restrict__ A, int32_t 
 #pragma unroll
     for (int k = 0; k < T; ++k) st[pos][k] = r[k];
     piv[K * T + pos] = lane;
+    if (PERT && npert > 0 && lane == 0) atomicAdd(amax + 1, (unsigned long long)npert);
   }
   __syncthreads();
   if (on) {
@@ -195,9 +216,10 @@ __device__ __forceinline__ void diag_core(int64_t K, S *__restrict__ A, int32_t 
   }
 }
 
-template <class S>
+template <class S, bool PERT = false>
 __device__ __forceinline__ void diag_body(int64_t K, int tb, S *__restrict__ band,
-                                          int32_t *__restrict__ piv) {
+                                          int32_t *__restrict__ piv,
+                                          unsigned long long *amax = nullptr) {
   __shared__ S st[T][T + 1];
   __shared__ S u12[16][T];
   __shared__ S l11[16][16];
@@ -206,13 +228,31 @@ __device__ __forceinline__ void diag_body(int64_t K, int tb, S *__restrict__ ban
 #pragma unroll 8
   for (int i = 0; i < T; ++i) st[i][lane] = A[i * T + lane];
   __syncthreads();
-  diag_core<S>(K, A, piv, st, u12, l11);
+  diag_core<S, PERT>(K, A, piv, st, u12, l11, amax);
 }
 
-template <class S>
+template <class S, bool PERT>
 __global__ __launch_bounds__(64) void k_diag(int64_t K, int tb, S *__restrict__ band,
-                                             int32_t *__restrict__ piv) {
-  diag_body<S>(K, tb, band, piv);
+                                             int32_t *__restrict__ piv,
+                                             unsigned long long *__restrict__ amax) {
+  diag_body<S, PERT>(K, tb, band, piv, amax);
+}
+
+// amax[0] = max |a| over n values (non-negative doubles order like their bits)
+template <class S>
+__global__ __launch_bounds__(256) void k_absmax(int64_t n, const S *__restrict__ a,
+                                                unsigned long long *__restrict__ amax) {
+  double m = 0.0;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256)
+    m = fmax(m, fabs((double)a[t]));
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0 && m > 0.0)
+    atomicMax(amax, (unsigned long long)__double_as_longlong(m));
+}
+
+__global__ void k_zero_u64(unsigned long long *__restrict__ p) {
+  if (threadIdx.x == 0) p[0] = 0;
 }
 
 // Panel of step K, one wavefront per tile.  kind 0: U_KJ = L_KK^-1 P_K A_KJ
@@ -445,9 +485,10 @@ __global__ __launch_bounds__(256) void k_update(int64_t K, int64_t nt, int tb, S
 // Trailing update of step K with the next diagonal LU fused in: block 0
 // updates tile (K+1, K+1) into LDS, then its first wavefront factors it
 // while the other blocks are still updating (requires K + 1 < nt).
-template <class S>
+template <class S, bool PERT>
 __global__ __launch_bounds__(256) void k_update_diag(int64_t K, int64_t nt, int tb,
-                                                     S *__restrict__ band, int32_t *__restrict__ piv) {
+                                                     S *__restrict__ band, int32_t *__restrict__ piv,
+                                                     unsigned long long *__restrict__ amax) {
   __shared__ __align__(16) S lds[MM_LDS];
   const bool fused = blockIdx.x == 0;
   update_body<S>((int64_t)blockIdx.x, K, nt, tb, band, lds, fused);
@@ -456,7 +497,7 @@ __global__ __launch_bounds__(256) void k_update_diag(int64_t K, int64_t nt, int 
   S(*st)[T + 1] = reinterpret_cast<S(*)[T + 1]>(lds);
   S(*u12)[T] = reinterpret_cast<S(*)[T]>(lds + T * (T + 1));
   S(*l11)[16] = reinterpret_cast<S(*)[16]>(lds + T * (T + 1) + 16 * T);
-  diag_core<S>(K + 1, tile_ptr(band, tb, K + 1, K + 1), piv, st, u12, l11);
+  diag_core<S, PERT>(K + 1, tile_ptr(band, tb, K + 1, K + 1), piv, st, u12, l11, amax);
 }
 
 // ------------------------------------------------- post-factorization pass
@@ -1105,20 +1146,27 @@ static void fill_t(Ctx &c, S *band) {
   MAFEM_HIP(hipGetLastError());
 }
 
-template <class S>
-static void factor_t(Ctx &c, S *band, S *li, S *ui) {
+template <class S, bool PERT>
+static void factor_run(Ctx &c, S *band, S *li, S *ui) {
   // the diagonal LU of step K + 1 runs inside step K's trailing update
   // (k_update_diag); one launch in the middle of the chain is timed
   const int64_t timed_K = c.nt / 2;
-  hipLaunchKernelGGL(k_diag<S>, dim3(1), dim3(64), 0, c.stream, (int64_t)0, c.tb, band, c.d_piv);
+  if (PERT) {  // max |K| of this tangent for the static-pivoting threshold
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c.stream, c.d_amax);
+    const int64_t nb = c.nnzb * c.ndf * c.ndf;
+    hipLaunchKernelGGL(k_absmax<double>, dim3(1024), dim3(256), 0, c.stream, nb,
+                       (const double *)c.d_bsr, c.d_amax);
+  }
+  hipLaunchKernelGGL((k_diag<S, PERT>), dim3(1), dim3(64), 0, c.stream, (int64_t)0, c.tb, band,
+                     c.d_piv, c.d_amax);
   for (int64_t K = 0; K < c.nt; ++K) {
     hipLaunchKernelGGL(k_panel<S>, dim3(3 * c.tb), dim3(64), 0, c.stream, K, c.nt, c.tb, band,
                        c.d_piv);
     if (K + 1 >= c.nt) break;  // no trailing tiles after the last step
     const bool timed = (K == timed_K);
     if (timed) MAFEM_HIP(hipEventRecord(c.ev[10], c.stream));
-    hipLaunchKernelGGL(k_update_diag<S>, dim3(c.tb * c.tb), dim3(256), 0, c.stream, K, c.nt, c.tb,
-                       band, c.d_piv);
+    hipLaunchKernelGGL((k_update_diag<S, PERT>), dim3(c.tb * c.tb), dim3(256), 0, c.stream, K,
+                       c.nt, c.tb, band, c.d_piv, c.d_amax);
     if (timed) {
       MAFEM_HIP(hipEventRecord(c.ev[11], c.stream));
       c.step_timed = true;
@@ -1205,12 +1253,17 @@ void band_fill(Ctx &c, int prec) {
     fill_t<double>(c, (double *)c.d_band);
 }
 
-void band_factor(Ctx &c) {
+void band_factor(Ctx &c, bool perturb) {
   ++c.n_factor;
-  if (c.band_prec == 32)
-    factor_t<float>(c, (float *)c.d_band, (float *)c.d_linv, (float *)c.d_uinv);
-  else
-    factor_t<double>(c, (double *)c.d_band, (double *)c.d_linv, (double *)c.d_uinv);
+  if (c.band_prec == 32) {
+    if (perturb) factor_run<float, true>(c, (float *)c.d_band, (float *)c.d_linv, (float *)c.d_uinv);
+    else factor_run<float, false>(c, (float *)c.d_band, (float *)c.d_linv, (float *)c.d_uinv);
+  } else {
+    if (perturb)
+      factor_run<double, true>(c, (double *)c.d_band, (double *)c.d_linv, (double *)c.d_uinv);
+    else
+      factor_run<double, false>(c, (double *)c.d_band, (double *)c.d_linv, (double *)c.d_uinv);
+  }
 }
 
 void band_solve(Ctx &c, const double *d_rhs, double *d_x) {

@@ -134,6 +134,11 @@ struct Ctx {
   bool sweep_tuned = false;      // the autotuner has chosen sweep_groups
   double *d_s1 = nullptr, *d_s2 = nullptr;  // sweep outputs (sentinel-filled, atomic access only)
   int *d_sweep_err = nullptr;    // count of sweep waits that ran out (must stay 0)
+  // static pivoting of the single-context factorization: d_amax[0] = max |K|
+  // (as the bits of a non-negative double), d_amax[1] = count of pivots
+  // replaced by +-sqrt(eps) max |K|
+  unsigned long long *d_amax = nullptr;
+  bool perturb_pivots = false;   // set once a factorization without it failed
 
   // staging of mafem_set_prescribed (grown on demand, kept)
   int64_t *d_presc_idx = nullptr;
@@ -296,7 +301,9 @@ void launch_output(Ctx &c);
 void launch_predict(Ctx &c, const double *d_u, double dt);
 // band LU and solves (band.hip)
 void band_fill(Ctx &c, int prec);
-void band_factor(Ctx &c);
+// perturb: static pivoting (diagonal-tile pivots below eps max |K| replaced
+// by +-eps max |K|), used when a factorization without it failed
+void band_factor(Ctx &c, bool perturb = false);
 void band_solve(Ctx &c, const double *d_rhs, double *d_x);
 void bsr_spmv(Ctx &c, const double *d_x, double *d_y);
 void vec_axpby(Ctx &c, int64_t n, double a, const double *x, double b, double *y);

@@ -55,7 +55,7 @@ class Stats(ctypes.Structure):
         ("n_factor", ctypes.c_int32), ("gmres_iters", ctypes.c_int32),
         ("step_kernel_us", ctypes.c_double),
         ("last_relres", ctypes.c_double),
-        ("sweep_groups", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("sweep_groups", ctypes.c_int32), ("pivot_perturbations", ctypes.c_int32),
     ]
 
 
