@@ -1000,11 +1000,42 @@ template <class S>
 __global__ __launch_bounds__(64) void kb_diag(StepMap sm, int64_t K, const int32_t *__restrict__ order,
                                               const PatchDev *__restrict__ pd,
                                               const uint8_t *__restrict__ on,
-                                              const uint8_t *__restrict__ fac) {
+                                              const uint8_t *__restrict__ fac,
+                                              const uint8_t *__restrict__ pert) {
   int local, tb;
   const int p = step_locate<0>(sm, order, local, tb);
   if (!fac_on(on, fac, p) || pd[p].prec != prec_of<S>()) return;
-  diag_body<S>(K, tb, (S *)pd[p].band, pd[p].piv);
+  if (pert[p])
+    diag_body<S, true>(K, tb, (S *)pd[p].band, pd[p].piv, pd[p].amax);
+  else
+    diag_body<S, false>(K, tb, (S *)pd[p].band, pd[p].piv);
+}
+
+// max |K| of every patch of precision S that factors with static pivoting,
+// before its factorization: zeroed, then reduced over its block-sparse values
+template <class S>
+__global__ void kb_amax_zero(int P, const PatchDev *__restrict__ pd,
+                             const uint8_t *__restrict__ on, const uint8_t *__restrict__ fac,
+                             const uint8_t *__restrict__ pert) {
+  const int p = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (p < P && fac_on(on, fac, p) && pert[p] && pd[p].prec == prec_of<S>()) pd[p].amax[0] = 0;
+}
+
+template <class S>
+__global__ __launch_bounds__(256) void kb_amax(Seg sg, const PatchDev *__restrict__ pd,
+                                               const uint8_t *__restrict__ on,
+                                               const uint8_t *__restrict__ fac,
+                                               const uint8_t *__restrict__ pert) {
+  const int p = sg.map[blockIdx.x];
+  if (!fac_on(on, fac, p) || !pert[p] || pd[p].prec != prec_of<S>()) return;
+  const PatchDev &d = pd[p];
+  const int64_t n = d.nnzb * d.ndf * d.ndf;
+  const int64_t t = (blockIdx.x - sg.blk0[p]) * 256 + threadIdx.x;
+  double m = t < n ? fabs(d.bsr[t]) : 0.0;
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0 && m > 0.0)
+    atomicMax(d.amax, (unsigned long long)__double_as_longlong(m));
 }
 
 template <class S>
@@ -1282,6 +1313,10 @@ static void b_factor_t(BatchPlan &B, const std::vector<std::vector<StepMap>> &st
   hipLaunchKernelGGL(kb_bsr_to_band<S>, dim3(B.bsr.nblk), dim3(256), 0, B.stream, B.bsr, B.d_pd,
                      f.on, f.fac);
   hipLaunchKernelGGL(kb_band_pad<S>, dim3(B.P), dim3(64), 0, B.stream, B.d_pd, f.on, f.fac);
+  hipLaunchKernelGGL(kb_amax_zero<S>, dim3((unsigned)((B.P + 255) / 256)), dim3(256), 0,
+                     B.stream, B.P, B.d_pd, f.on, f.fac, f.pert);
+  hipLaunchKernelGGL(kb_amax<S>, dim3(B.bsr.nblk), dim3(256), 0, B.stream, B.bsr, B.d_pd, f.on,
+                     f.fac, f.pert);
   MAFEM_HIP(hipGetLastError());
   for (int64_t K = 0; K < (int64_t)steps.size(); ++K)
     for (const StepMap &sm : steps[K]) {
@@ -1297,7 +1332,7 @@ static void b_factor_t(BatchPlan &B, const std::vector<std::vector<StepMap>> &st
       smp.blk0[sm.ncls] = bp;
       smu.blk0[sm.ncls] = bu;
       hipLaunchKernelGGL(kb_diag<S>, dim3(nd), dim3(64), 0, B.stream, sm, K, B.d_order, B.d_pd,
-                         f.on, f.fac);
+                         f.on, f.fac, f.pert);
       hipLaunchKernelGGL(kb_panel<S>, dim3(bp), dim3(64), 0, B.stream, smp, K, B.d_order, B.d_pd,
                          f.on, f.fac);
       if (bu > 0)

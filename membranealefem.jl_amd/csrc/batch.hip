@@ -125,6 +125,7 @@ PatchDev describe(const Ctx &c) {
   d.s1 = c.d_s1;
   d.s2 = c.d_s2;
   d.err = c.d_sweep_err;
+  d.amax = c.d_amax;
   return d;
 }
 
@@ -285,6 +286,7 @@ void alloc_flags(mafem_batch &b) {
   f.kon = falloc<uint8_t>(b, P);
   f.fresh = falloc<uint8_t>(b, P);
   f.esc = falloc<uint8_t>(b, P);
+  f.pert = falloc<uint8_t>(b, P);
   f.iters = falloc<int32_t>(b, P);
   f.kits = falloc<int32_t>(b, P);
   f.kdim = falloc<int32_t>(b, P);
@@ -305,7 +307,7 @@ void alloc_flags(mafem_batch &b) {
   f.count = falloc<int32_t>(b, 8);
   const std::vector<uint8_t> zeros8(P, 0), ones8(P, 1);
   const std::vector<int32_t> zeros32(std::max(P, 8), 0);
-  for (uint8_t *a : {f.on, f.sol, f.kon, f.fresh, f.esc})
+  for (uint8_t *a : {f.on, f.sol, f.kon, f.fresh, f.esc, f.pert})
     MAFEM_HIP(hipMemcpy(a, zeros8.data(), P, hipMemcpyHostToDevice));
   MAFEM_HIP(hipMemcpy(f.fac, ones8.data(), P, hipMemcpyHostToDevice));  // no factor yet
   for (int32_t *a : {f.iters, f.kits, f.kdim, f.nfac, f.kits_tot})
@@ -566,7 +568,11 @@ int mafem_batch_time_step(mafem_batch *bp, const double *time, const double *dt,
       MAFEM_HIP(hipEventRecord(b.ev[1], B.stream));
       run(b, b.g_asm, issue_asm);
       MAFEM_HIP(hipEventRecord(b.ev[2], B.stream));
-      for (int attempt = 0; attempt < 3; ++attempt) {
+      // a failed solve moves a patch one rung up: stale factorization ->
+      // fresh one -> FP64 storage -> static pivoting (further cycles on the
+      // same factors, up to an Arnoldi-step budget) -> hard failure; the
+      // bound here only guarantees the loop ends
+      for (int attempt = 0; attempt < 32; ++attempt) {
         if (pend > 0) {
           MAFEM_HIP(hipEventRecord(b.ev[3], B.stream));
           run(b, b.g_fac, issue_fac);

@@ -200,6 +200,7 @@ struct PatchDev {
   int32_t sweep_w;   // workgroups of its triangular sweeps
   double *s1, *s2;   // forward/backward outputs of multi-workgroup sweeps (atomic access only)
   int *err;          // count of sweep waits that ran out
+  unsigned long long *amax;  // static pivoting: max |K| bits, replaced-pivot count
 };
 
 // Block -> patch map of one batched launch.
@@ -247,6 +248,7 @@ struct BatchFlags {
   double *part;      // [vec blocks] per-block partial sums
   double *part2;     // [m+1][vec blocks] per-block partial dot products
   uint8_t *esc;      // FP32 factorization could not precondition the solve: move to FP64
+  uint8_t *pert;     // factor with static pivoting (set after a fresh FP64 factorization failed)
   int32_t *count;    // [8] patches still on, solves unconverged, factorizations pending,
                      // hard failures, precision escalations
 };

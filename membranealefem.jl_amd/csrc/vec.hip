@@ -532,9 +532,16 @@ __global__ __launch_bounds__(256) void kb_count_sol(int P, BatchFlags f) {
 // Unconverged solves after the cycles: with a stale factorization, refactor
 // and continue from the current iterate; with a fresh FP32 one, move the
 // patch to FP64 storage (esc) and continue after refactoring; with a fresh
-// FP64 one, keep the inexact correction and count a hard failure.
+// FP64 one, refactor with static pivoting (pert, kept for the rest of the
+// batch's life) and continue; with a fresh FP64 one that already pivots
+// statically, continue on the same factors until the solve has taken
+// kPertArnoldi Arnoldi steps (its replaced pivots leave a few outlying
+// eigenvalues that GMRES needs several restart cycles to resolve), then keep
+// the inexact correction and count a hard failure.
 // count[1] = solves to continue, count[2] = factorizations pending,
 // count[4] = escalations.
+constexpr int32_t kPertArnoldi = 400;
+
 __global__ __launch_bounds__(256) void kb_solve_end(int P, const PatchDev *__restrict__ pd,
                                                     BatchFlags f) {
   int redo = 0, pend = 0, hard = 0, esc = 0;
@@ -548,6 +555,12 @@ __global__ __launch_bounds__(256) void kb_solve_end(int P, const PatchDev *__res
         f.fac[p] = 1;
         ++redo;
         ++esc;
+      } else if (!f.pert[p]) {  // fresh FP64: refactor once more, with static pivoting
+        f.pert[p] = 1;
+        f.fac[p] = 1;
+        ++redo;
+      } else if (f.kits[p] < kPertArnoldi) {  // statically pivoted: more cycles, same factors
+        ++redo;
       } else {
         f.sol[p] = 0;
         ++hard;

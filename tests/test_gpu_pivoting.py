@@ -38,6 +38,41 @@ def test_zero_pivot_tangent_converges_with_static_pivoting():
     assert st["pivot_perturbations"] > 0
 
 
+@pytest.mark.parametrize("precision", [32, 64])
+def test_batched_zero_pivot_member_converges_with_static_pivoting(precision):
+    """The same request as a member of a batch: its fresh FP64 factorization
+    (after the FP32 one moved it to FP64) fails, the member refactors with
+    static pivoting, GMRES runs the cycles the replaced pivots need (about 100
+    Arnoldi steps) and the member takes the reference's 3 Newton iterations
+    with no inexact correction; the other member keeps the result it has
+    alone."""
+    print(f"static pivoting: batched 96x96 EUL request, precision {precision}", flush=True)
+    specs = serving_specs(24)
+    pts = [Patch(specs[12]), Patch(specs[0])]
+    solo = Patch(specs[0])
+    try:
+        for pt in pts + [solo]:
+            pt.set_pull(pt.spec.speed_at(0))
+        b = mafem.Batch([pt.dev for pt in pts], precision=precision)
+        try:
+            it, eps, st = b.time_step(np.full(2, 1e-3), 1e-3, predict=True)
+            hard = b.stats()["hard_failures"]
+        finally:
+            b.close()
+        b1 = mafem.Batch([solo.dev], precision=precision)
+        try:
+            it1, eps1, st1 = b1.time_step(np.full(1, 1e-3), 1e-3, predict=True)
+        finally:
+            b1.close()
+    finally:
+        for pt in pts + [solo]:
+            pt.close()
+    print(f"  iters {list(it)} eps {eps[0]}", flush=True)
+    assert list(st) == [0, 0] and it[0] == 3 and hard == 0
+    assert eps[0][-1] < 1e-10 and abs(eps[0][1] - 3.76e-5) < 1e-7
+    assert it[1] == it1[0] and np.allclose(eps[1], eps1[0], rtol=1e-6, atol=1e-13)
+
+
 def test_well_posed_tangent_uses_no_perturbation():
     p = mafem.Params(motion=mafem.ALEVB, num1el=24, num2el=24, length=1.0, enr=1e-10,
                      output=False)
