@@ -224,7 +224,10 @@ typedef struct mafem_batch mafem_batch;
 typedef struct {
   int32_t factor_precision;  /* 32 (default) or 64 */
   int32_t restart;           /* GMRES basis size m (default 20, at most 64) */
-  int32_t max_cycles;        /* restart cycles per solve before refactoring (default 1) */
+  int32_t max_cycles;        /* restart cycles per solve before refactoring (default 1); a
+                                failed solve moves its member up one rung: stale factors ->
+                                fresh -> FP64 storage -> static pivoting, where it keeps
+                                cycling until the solve has taken 400 Arnoldi steps */
   int32_t refactor_iters;    /* refactor when a solve needed more Arnoldi steps (default 6;
                                 -1: factor afresh for every Newton iteration) */
   int32_t use_graphs;        /* 1 (default): replay recorded HIP graphs */
@@ -238,7 +241,8 @@ typedef struct {
   int32_t newton_iters;      /* largest per-patch Newton iteration count */
   int32_t factor_passes;     /* batched factorization passes */
   int32_t gmres_cycles;      /* batched GMRES restart cycles */
-  int32_t hard_failures;     /* solves left unconverged after a fresh factorization */
+  int32_t hard_failures;     /* solves left unconverged on fresh, statically pivoted FP64
+                                factors after 400 Arnoldi steps (the correction is kept) */
   int64_t patch_iters;       /* sum over patches of Newton iterations */
   int64_t patch_factors;     /* sum over patches of factorizations */
   int64_t gmres_iters;       /* sum over patches of Arnoldi steps */

@@ -289,11 +289,13 @@ __device__ __forceinline__ void panel_body(int64_t bid, int64_t K, int64_t nt, i
   const int64_t IJ = K + 1 + off;
   if (IJ >= nt) return;
   const S *D = tile_ptr(band, tb, K, K);
-#pragma unroll 8
+  // full unrolling keeps all 64 row loads of a tile in flight (one wave per
+  // tile: grouped loads would pay the memory latency once per group)
+#pragma unroll
   for (int i = 0; i < T; ++i) sD[i][lane] = D[i * T + lane];
   if (kind == 0) {
     S *A = tile_ptr(band, tb, K, IJ);
-#pragma unroll 8
+#pragma unroll
     for (int i = 0; i < T; ++i) sX[i][lane] = A[(int64_t)pv[i] * T + lane];
     __syncthreads();
     // x_i -= D_ik x_k for k < i, k increasing (unit lower L)
@@ -323,7 +325,7 @@ __device__ __forceinline__ void panel_body(int64_t bid, int64_t K, int64_t nt, i
     for (int i = 0; i < T; ++i) A[i * T + lane] = sX[i][lane];
   } else {
     S *A = tile_ptr(band, tb, IJ, K);
-#pragma unroll 8
+#pragma unroll
     for (int i = 0; i < T; ++i) sX[i][lane] = A[i * T + lane];
     __syncthreads();
     rdiag[lane] = S(1) / sD[lane][lane];
@@ -643,6 +645,37 @@ __device__ __forceinline__ double band_row_sum(const S *band, int tb, int64_t I,
   return acc;
 }
 
+// The same sum over K = K1, K1 - 1, ..., K0 (decreasing), for the backward sweep.
+template <class S, int B = (sizeof(S) == 4 ? 24 : 10)>
+__device__ __forceinline__ double band_row_sum_desc(const S *band, int tb, int64_t I, int64_t K1,
+                                                    int64_t K0, const double (*ring)[T],
+                                                    double acc = 0.0) {
+  const int i = threadIdx.x >> 4, q = threadIdx.x & 15;
+  for (int64_t Kb = K1; Kb >= K0; Kb -= B) {
+    typename Vec4<S>::type m[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u)
+      if (Kb - u >= K0)
+        m[u] = *reinterpret_cast<const typename Vec4<S>::type *>(
+            tile_ptr(band, tb, I, Kb - u) + i * T + 4 * q);
+#pragma unroll
+    for (int u = 0; u < B; ++u)
+      if (Kb - u >= K0) {
+        const double *x = ring[(Kb - u) % RING] + 4 * q;
+        acc = fma((double)m[u].x, x[0], acc);
+        acc = fma((double)m[u].y, x[1], acc);
+        acc = fma((double)m[u].z, x[2], acc);
+        acc = fma((double)m[u].w, x[3], acc);
+      }
+  }
+  return acc;
+}
+
+template <class S>
+__device__ __forceinline__ double tile_row_fma(const typename Vec4<S>::type &m, const double *x) {
+  return fma((double)m.x, x[0], fma((double)m.y, x[1], fma((double)m.z, x[2], (double)m.w * x[3])));
+}
+
 // out_I = M_I v_I for every tile row (M = L^-1 with v = P b, or U^-1 with v = y)
 template <class S>
 __device__ __forceinline__ void diag_apply_body(int64_t bid, const S *__restrict__ inv,
@@ -686,7 +719,10 @@ __global__ __launch_bounds__(256) void k_diag_apply_atomic(const S *__restrict__
   diag_apply_body<S>((int64_t)blockIdx.x, inv, (const int32_t *)nullptr, v, out, true);
 }
 
-// y_I = c_I - Σ_{K<I} L~_IK y_K  (c precomputed into y)
+// y_I = c_I - Σ_{K<I} L~_IK y_K  (c precomputed into y).  Tiles K0 .. I - 2
+// are summed in increasing order into one chain, the neighbour tile I - 1 on
+// its own and added last: the order of the multi-workgroup sweep, so the
+// result does not depend on the number of workgroups.
 template <class S>
 __device__ __forceinline__ void fwd_sweep_body(int64_t nt, int tb, const S *__restrict__ band,
                                                double *__restrict__ y) {
@@ -694,7 +730,12 @@ __device__ __forceinline__ void fwd_sweep_body(int64_t nt, int tb, const S *__re
   const int i = threadIdx.x >> 4, q = threadIdx.x & 15;
   for (int64_t I = 0; I < nt; ++I) {
     const int64_t K0 = I - tb < 0 ? 0 : I - tb;
-    const double s = row16_sum(band_row_sum(band, tb, I, K0, I, ring));
+    double acc = band_row_sum(band, tb, I, K0, I - 1, ring);
+    if (I >= 1)
+      acc += tile_row_fma<S>(*reinterpret_cast<const typename Vec4<S>::type *>(
+                                 tile_ptr(band, tb, I, I - 1) + i * T + 4 * q),
+                             ring[(I - 1) % RING] + 4 * q);
+    const double s = row16_sum(acc);
     if (q == 0) {
       const double u = y[I * T + i] - s;
       ring[I % RING][i] = u;
@@ -704,7 +745,9 @@ __device__ __forceinline__ void fwd_sweep_body(int64_t nt, int tb, const S *__re
   }
 }
 
-// x_I = d_I - Σ_{J>I} U~_IJ x_J  (d precomputed into x)
+// x_I = d_I - Σ_{J>I} U~_IJ x_J  (d precomputed into x).  Tiles J1 .. I + 2
+// are summed in decreasing order into one chain, the neighbour tile I + 1 on
+// its own and added last (the order of the multi-workgroup sweep).
 template <class S>
 __device__ __forceinline__ void bwd_sweep_body(int64_t nt, int tb, const S *__restrict__ band,
                                                double *__restrict__ x) {
@@ -712,7 +755,12 @@ __device__ __forceinline__ void bwd_sweep_body(int64_t nt, int tb, const S *__re
   const int i = threadIdx.x >> 4, q = threadIdx.x & 15;
   for (int64_t I = nt - 1; I >= 0; --I) {
     const int64_t J1 = I + tb > nt - 1 ? nt - 1 : I + tb;
-    const double s = row16_sum(band_row_sum(band, tb, I, I + 1, J1 + 1, ring));
+    double acc = band_row_sum_desc<S, sizeof(S) == 4 ? 24 : 8>(band, tb, I, J1, I + 2, ring);
+    if (I + 1 <= nt - 1)
+      acc += tile_row_fma<S>(*reinterpret_cast<const typename Vec4<S>::type *>(
+                                 tile_ptr(band, tb, I, I + 1) + i * T + 4 * q),
+                             ring[(I + 1) % RING] + 4 * q);
+    const double s = row16_sum(acc);
     if (q == 0) {
       const double u = x[I * T + i] - s;
       ring[I % RING][i] = u;
@@ -779,11 +827,6 @@ __device__ __forceinline__ void stage_rows(const unsigned long long *__restrict_
     }
     ring[K % RING][lane] = __longlong_as_double((long long)v);
   }
-}
-
-template <class S>
-__device__ __forceinline__ double tile_row_fma(const typename Vec4<S>::type &m, const double *x) {
-  return fma((double)m.x, x[0], fma((double)m.y, x[1], fma((double)m.z, x[2], (double)m.w * x[3])));
 }
 
 // y_I = c_I - Σ_{K<I} L~_IK y_K
@@ -872,15 +915,35 @@ __device__ __forceinline__ void bwd_sweep_mw_body(int w, int W, int64_t nt, int 
       ml = *reinterpret_cast<const typename Vec4<S>::type *>(tile_ptr(band, tb, I, I + 1) + i * T +
                                                              4 * q);
     double acc = 0.0;
-    if (I + 1 < J1) {
-      // rows J1 down to I + 2, staged from the highest not yet in the ring
+    // rows from I + W up are published (this workgroup's previous row and all
+    // it staged before): their tiles are summed without waiting, highest
+    // first; rows I + W - 1 .. I + 2 follow one at a time, each as it
+    // appears, in the same decreasing order, so the sum does not depend on W
+    const int64_t Ja = I + W > I + 2 ? I + W : I + 2;
+    if (Ja <= J1) {
       const int64_t hi = have - 1 < J1 ? have - 1 : J1;
-      stage_rows(x, I + 2, hi, ring, &bad);
-      if (I + 2 < have) have = I + 2;
+      stage_rows(x, Ja, hi, ring, &bad);
+      if (Ja < have) have = Ja;
       __syncthreads();
       if (bad) break;
-      acc = band_row_sum<S, sizeof(S) == 4 ? 16 : 8>(band, tb, I, I + 2, J1 + 1, ring);
+      acc = band_row_sum_desc<S, sizeof(S) == 4 ? 12 : 8>(band, tb, I, J1, Ja, ring);
     }
+    for (int64_t r = (Ja - 1 < J1 ? Ja - 1 : J1); r >= I + 2; --r) {
+      const typename Vec4<S>::type m =
+          *reinterpret_cast<const typename Vec4<S>::type *>(tile_ptr(band, tb, I, r) + i * T + 4 * q);
+      if (r < have) {
+        stage_rows(x, r, r, ring, &bad);
+        have = r;
+      }
+      __syncthreads();
+      if (bad) break;
+      const double *xr = ring[r % RING] + 4 * q;
+      acc = fma((double)m.x, xr[0], acc);
+      acc = fma((double)m.y, xr[1], acc);
+      acc = fma((double)m.z, xr[2], acc);
+      acc = fma((double)m.w, xr[3], acc);
+    }
+    if (bad) break;
     if (I + 1 <= nt - 1) {
       if (I + 1 < have) {
         stage_rows(x, I + 1, I + 1, ring, &bad);
