@@ -33,8 +33,8 @@ __device__ __forceinline__ S *tile_ptr(S *band, int tb, int64_t I, int64_t J) {
 
 template <class S>
 __global__ __launch_bounds__(256) void k_zero_t(int64_t n, S *__restrict__ x) {
-  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (t < n) x[t] = S(0);
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256)
+    x[t] = S(0);
 }
 
 template <class S>
@@ -1229,7 +1229,10 @@ static inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
 template <class S>
 static void fill_t(Ctx &c, S *band) {
   const int64_t nband = c.nt * (2 * c.tb + 1) * (int64_t)T * T;
-  hipLaunchKernelGGL(k_zero_t<S>, dim3(nblk(nband)), dim3(256), 0, c.stream, nband, band);
+  // grid-stride over a capped grid: a dispatch counts its work-items in 32
+  // bits, and a large patch's band (256x256 ALEVB: 4.5e9 entries) exceeds that
+  const unsigned zb = (unsigned)std::min<int64_t>(nblk(nband), 1 << 20);
+  hipLaunchKernelGGL(k_zero_t<S>, dim3(zb), dim3(256), 0, c.stream, nband, band);
   const int64_t nb = c.nnzb * c.ndf * c.ndf;
   hipLaunchKernelGGL(k_bsr_to_band<S>, dim3(nblk(nb)), dim3(256), 0, c.stream, c.nnzb, c.ndf, c.tb,
                      c.d_brow, c.d_colidx, c.d_bsr, band);

@@ -296,6 +296,7 @@ void alloc_flags(mafem_batch &b) {
   f.dt = falloc<double>(b, P);
   f.bn = falloc<double>(b, P);
   f.relres = falloc<double>(b, P);
+  f.relres0 = falloc<double>(b, P);
   f.eps = falloc<double>(b, P);
   f.scale = falloc<double>(b, P);
   f.H = falloc<double>(b, (int64_t)P * m * (m + 1));

@@ -240,6 +240,7 @@ struct BatchFlags {
   int32_t *kits_tot; // Arnoldi steps (running count)
   double *time, *dt; // per patch
   double *bn, *relres, *eps, *scale;
+  double *relres0;   // true relative residual at the start of the last restart cycle
   double *H;         // [P][m][m+1]: column j of the Hessenberg matrix at H + (p m + j)(m+1)
   double *cs, *sn;   // [P][m] Givens rotations
   double *g;         // [P][m+1] rotated right-hand side

@@ -331,6 +331,7 @@ __global__ __launch_bounds__(256) void kb_cycle_start(Seg sg, const PatchDev *__
     const double beta = sqrt(s);
     const double rr = beta / f.bn[p];
     f.relres[p] = rr;
+    f.relres0[p] = rr;
     f.kon[p] = rr > rtol ? 1 : 0;
     f.kdim[p] = 0;
     double *g = f.g + (int64_t)p * (m + 1);
@@ -534,10 +535,12 @@ __global__ __launch_bounds__(256) void kb_count_sol(int P, BatchFlags f) {
 // patch to FP64 storage (esc) and continue after refactoring; with a fresh
 // FP64 one, refactor with static pivoting (pert, kept for the rest of the
 // batch's life) and continue; with a fresh FP64 one that already pivots
-// statically, continue on the same factors until the solve has taken
-// kPertArnoldi Arnoldi steps (its replaced pivots leave a few outlying
-// eigenvalues that GMRES needs several restart cycles to resolve), then keep
-// the inexact correction and count a hard failure.
+// statically, continue on the same factors while each cycle at least halves
+// the true residual, up to kPertArnoldi Arnoldi steps (its replaced pivots
+// leave a few outlying eigenvalues that GMRES needs several restart cycles
+// to resolve; a stalled solve, as on the tangent of a diverging Newton
+// iteration, stops at once), then keep the inexact correction and count a
+// hard failure.
 // count[1] = solves to continue, count[2] = factorizations pending,
 // count[4] = escalations.
 constexpr int32_t kPertArnoldi = 400;
@@ -559,7 +562,8 @@ __global__ __launch_bounds__(256) void kb_solve_end(int P, const PatchDev *__res
         f.pert[p] = 1;
         f.fac[p] = 1;
         ++redo;
-      } else if (f.kits[p] < kPertArnoldi) {  // statically pivoted: more cycles, same factors
+      } else if (f.kits[p] < kPertArnoldi && f.relres[p] < 0.5 * f.relres0[p]) {
+        // statically pivoted and still converging: another cycle on the same factors
         ++redo;
       } else {
         f.sol[p] = 0;
