@@ -286,6 +286,28 @@ def test_multi_workgroup_sweeps_larger_patches(motion, n, groups, monkeypatch):
     assert np.array_equal(sols[1], sols[0])
 
 
+def test_direct_solve_on_a_band_beyond_one_dispatch():
+    """A 256x256 ALEVB patch: its FP64 band holds 8321 x 131 x 64^2 = 4.5e9
+    entries (36 GB), more than a dispatch's 32-bit work-item count.  The band
+    is zeroed by a grid-stride kernel over a capped grid, and the direct solve
+    of the Newton right-hand side is finite and meets the refinement target."""
+    print("256x256 ALEVB direct solve", flush=True)
+    p = mafem.Params(motion=mafem.ALEVB, num1el=256, num2el=256, length=1.0, enr=1e-10,
+                     output=False)
+    mesh = mafem.make_mesh(p, pull_speed=0.3)
+    xms, cps = mafem.prepare_input(p, mesh, pull_speed=0.3)
+    dev = mafem.DeviceMesh(mesh, p)
+    try:
+        dev.set_state(xms, cps)
+        r, _ = dev.assemble(1e-3, 1e-3, want_K=False)
+        x, rel = dev.solve(-r)
+    finally:
+        dev.close()
+    print(f"  relres {rel:.2e}", flush=True)
+    assert np.isfinite(x).all() and np.abs(x).max() > 0.0
+    assert rel <= 1e-10
+
+
 def test_sweep_autotuner_changes_time_only(monkeypatch):
     """The sweep split is autotuned on the first factorization; every
     candidate gives bit-identical results, so a time step with the tuned
