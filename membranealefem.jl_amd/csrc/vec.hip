@@ -534,7 +534,9 @@ __global__ __launch_bounds__(256) void kb_count_sol(int P, BatchFlags f) {
 // and continue from the current iterate; with a fresh FP32 one, move the
 // patch to FP64 storage (esc) and continue after refactoring; with a fresh
 // FP64 one, refactor with static pivoting (pert, kept for the rest of the
-// batch's life) and continue; with a fresh FP64 one that already pivots
+// batch's life) and continue, unless the Newton iteration is diverging (its
+// last ε at least kPertMaxEps: a hard failure at once, since no factorization
+// helps a garbage tangent); with a fresh FP64 one that already pivots
 // statically, continue on the same factors while each cycle at least halves
 // the true residual, up to kPertArnoldi Arnoldi steps (its replaced pivots
 // leave a few outlying eigenvalues that GMRES needs several restart cycles
@@ -544,6 +546,7 @@ __global__ __launch_bounds__(256) void kb_count_sol(int P, BatchFlags f) {
 // count[1] = solves to continue, count[2] = factorizations pending,
 // count[4] = escalations.
 constexpr int32_t kPertArnoldi = 400;
+constexpr double kPertMaxEps = 0.1;  // last Newton ε above which the iteration is diverging
 
 __global__ __launch_bounds__(256) void kb_solve_end(int P, const PatchDev *__restrict__ pd,
                                                     BatchFlags f) {
@@ -558,7 +561,9 @@ __global__ __launch_bounds__(256) void kb_solve_end(int P, const PatchDev *__res
         f.fac[p] = 1;
         ++redo;
         ++esc;
-      } else if (!f.pert[p]) {  // fresh FP64: refactor once more, with static pivoting
+      } else if (!f.pert[p] && (f.iters[p] == 0 || f.eps[p] < kPertMaxEps)) {
+        // fresh FP64 and the Newton iteration not diverging: refactor once
+        // more, with static pivoting
         f.pert[p] = 1;
         f.fac[p] = 1;
         ++redo;
