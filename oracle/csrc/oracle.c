@@ -20,8 +20,10 @@
  * Arithmetic is ComplexF64 throughout, as in the reference, with step ek.
  * Pinned by: the closed forms of test/analysis/geo_dyn_stress_tests.jl
  * (paraboloid geometry, H, K; Couette/Poiseuille stress; area), the flat
- * equilibrium invariant r = 0, central finite differences of the residual,
- * and a Newton-convergence check (tests/test_oracle_*.py).
+ * equilibrium invariant r = 0, flat-patch closed forms of the mesh-motion
+ * rows (ALEV/ALEVB sigma_m, p^m and rp terms; the EUL rm projection) and of
+ * the tension row's surface divergence, central finite differences of the
+ * residual, and a Newton-convergence check (tests/test_oracle_*.py).
  */
 #include <complex.h>
 #include <math.h>

@@ -17,8 +17,12 @@ linear-solve rounding is "parity unpinned" against UMFPACK itself.
 
 Pinned by: the equilibrium invariant (flat F_PULL state at t = 0 with zero
 pull speed gives r = 0), central finite differences of the residual against
-the tangent, quadratic Newton convergence, and the closed-form GeoDynStress
-tests of the reference (tests/test_oracle_*.py).  The pull force is pinned by
+the tangent, quadratic Newton convergence, the closed-form GeoDynStress
+tests of the reference, flat-patch closed forms of the mesh-motion rows
+(ALEV/ALEVB rm from a linear v^m field and a constant p^m, rp from a normal
+v^m - v, EUL rm = am (v^m - (n.v) n) N J w, the tension row's surface
+divergence) and the ALEV bending steady state converging to the analytic arc
+under refinement (tests/test_oracle_*.py).  The pull force is pinned by
 its closed form on the flat tension state (f = (kb/4) L x the integral of the
 pulled element's edge basis functions for an edge element, 0 for an interior
 one), by the sum of every element's residual over the pulled nodes, and by

@@ -8,9 +8,11 @@ Follows (reference paths):
   src/input/Bc.jl      scenario BCs :15-393, get_dofs :414-431
   src/Input.jl         prepare_input :22-121, get_pull_el_id :131-136
 
-Pinned by: the DOF and nnz counts that follow from Bc.jl's arithmetic at the
-benchmark sizes (tests/golden/mesh_counts.json, derived by hand from the BC
-rules: boundary nodes, inner ring, pull element, edge-centre nodes), the
+Pinned by: the unknown counts in tests/golden/mesh_counts.json (4 x 4 rows
+counted by hand per unknown on the 6 x 6 node grid; nmdf at the benchmark
+sizes), the F_PULL Dirichlet sets rebuilt from grid coordinates (boundary
+ring, inner ring, the pulled element's 3 x 3 node block, edge-centre nodes)
+and compared entry by entry for every motion (tests/test_oracle_ale.py), the
 structure of IX (tensor-product node numbering), and agreement with the
 independently written product-side mesh builder.
 """
