@@ -444,7 +444,9 @@ void check_sweeps(Ctx &c) {
   MAFEM_HIP(hipStreamSynchronize(c.stream));
   if (e != 0) {
     MAFEM_HIP(hipMemsetAsync(c.d_sweep_err, 0, sizeof(int), c.stream));
-    throw ArgError{"a triangular sweep wait ran out (results discarded)", MAFEM_EHIP};
+    throw ArgError{"a triangular sweep wait ran out: the solves since the last check are "
+                   "invalid and so is the context state they updated (set it again "
+                   "before continuing)", MAFEM_EHIP};
   }
 }
 

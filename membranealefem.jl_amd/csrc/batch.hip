@@ -89,8 +89,15 @@ void free_plan(mafem_batch &b) {
   b.owned.clear();
 }
 
-// Segment map from per-patch block counts (0 = patch not in this launch).
-Seg make_seg(mafem_batch &b, const std::vector<int64_t> &blocks) {
+// Segment map from per-patch block counts (0 = patch not in this launch);
+// `threads` is the workgroup size of every kernel launched over the segment.
+Seg make_seg(mafem_batch &b, const std::vector<int64_t> &blocks, unsigned threads) {
+  uint64_t total = 0;
+  for (int64_t n : blocks) total += n > 0 ? (uint64_t)n : 0;
+  if (!launch_fits(total, threads))
+    throw BatchError{"batch too large for one launch: " + std::to_string(total) + " workgroups of " +
+                         std::to_string(threads) + " work-items exceed a dispatch's 2^32 - 1",
+                     MAFEM_EUNSUPPORTED};
   std::vector<int32_t> map;
   std::vector<int64_t> blk0(blocks.size(), -1);
   for (size_t p = 0; p < blocks.size(); ++p) {
@@ -98,7 +105,6 @@ Seg make_seg(mafem_batch &b, const std::vector<int64_t> &blocks) {
     blk0[p] = (int64_t)map.size();
     map.insert(map.end(), (size_t)blocks[p], (int32_t)p);
   }
-  if (map.size() > 0x7fffffffu) throw BatchError{"batch too large for one launch", MAFEM_EUNSUPPORTED};
   Seg s;
   s.nblk = (unsigned)map.size();
   s.map = bupload(b, map);
@@ -235,8 +241,8 @@ void build_plan(mafem_batch &b) {
       ba[p] = cdiv(d.numel * d.ne, 256);
       be[p] = cdiv(d.n_edges * d.ne, 256);
     }
-    B.area.push_back(make_seg(b, ba));
-    B.edge.push_back(make_seg(b, be));
+    B.area.push_back(make_seg(b, ba, 256));
+    B.edge.push_back(make_seg(b, be, 256));
   }
   std::vector<int64_t> bb(P), bv(P), bn(P), bz(P), bt(P), bf(P);
   for (int p = 0; p < P; ++p) {
@@ -248,15 +254,15 @@ void build_plan(mafem_batch &b) {
     bt[p] = d.nt;
     bf[p] = d.nt * (2 * d.tb + 1);
   }
-  B.bsr = make_seg(b, bb);
-  B.vec = make_seg(b, bv);
-  B.node = make_seg(b, bn);
-  B.zero = make_seg(b, bz);
-  B.tiles = make_seg(b, bt);
-  B.fold = make_seg(b, bf);
+  B.bsr = make_seg(b, bb, 256);
+  B.vec = make_seg(b, bv, 256);
+  B.node = make_seg(b, bn, 256);
+  B.zero = make_seg(b, bz, 256);
+  B.tiles = make_seg(b, bt, 256);  // tile inversion (64), diagonal products (256)
+  B.fold = make_seg(b, bf, 256);
   std::vector<int64_t> bw(P);
   for (int p = 0; p < P; ++p) bw[p] = b.h_pd[p].sweep_w > 1 ? b.h_pd[p].sweep_w : 0;
-  B.mw = make_seg(b, bw);
+  B.mw = make_seg(b, bw, 1024);  // multi-workgroup sweeps
 
   // factorization order: by precision, then classes of equal tb, each by
   // decreasing nt
@@ -664,7 +670,9 @@ int mafem_batch_time_step(mafem_batch *bp, const double *time, const double *dt,
       MAFEM_HIP(hipMemcpy(&e, b.h_pd[p].err, sizeof(int), hipMemcpyDeviceToHost));
       if (e != 0) {
         MAFEM_HIP(hipMemset(b.h_pd[p].err, 0, sizeof(int)));
-        throw BatchError{"a triangular sweep wait ran out (results discarded)", MAFEM_EHIP};
+        throw BatchError{"a triangular sweep wait ran out: this step's corrections are invalid "
+                         "and so are the member states they updated (set them again "
+                         "before continuing)", MAFEM_EHIP};
       }
     }
     // what each member's band holds now, for a later batch

@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 
+#include "launch_limits.h"
 #include "mafem.h"
 
 namespace mafem {

@@ -567,8 +567,9 @@ __global__ __launch_bounds__(256) void kb_solve_end(int P, const PatchDev *__res
         f.pert[p] = 1;
         f.fac[p] = 1;
         ++redo;
-      } else if (f.kits[p] < kPertArnoldi && f.relres[p] < 0.5 * f.relres0[p]) {
+      } else if (f.pert[p] && f.kits[p] < kPertArnoldi && f.relres[p] < 0.5 * f.relres0[p]) {
         // statically pivoted and still converging: another cycle on the same factors
+        // (a member that skipped the static-pivoting rung fails at once)
         ++redo;
       } else {
         f.sol[p] = 0;
