@@ -51,9 +51,13 @@ def pull_speed(k, total=200, final=2.0):
     return final * (k + 1) / total
 
 
-# k_update_diag HBM counters per dispatch (KiB, mean over the 2176 launches of
-# one time step of the 64x64 ALEVB bench patch); see the roofline below
-STEP_FETCH_KIB, STEP_WRITE_KIB = 8914.95051125919, 9064.25
+# HBM counters per dispatch of the factorization-step kernel (FETCH_SIZE,
+# WRITE_SIZE in KiB, mean over one time step's dispatches of the 64x64 ALEVB
+# bench patch), by kernel; see the roofline below.  None: not measured.
+STEP_COUNTERS_KIB = {
+    "k_update_diag": (8914.95051125919, 9064.25),
+    "k_update_diag2": None,
+}
 
 
 def band_update_flops(dev):
@@ -76,9 +80,12 @@ def dev_band_shape(dev):
     return nt, tb
 
 
-def cpu_baseline(p_gpu, mesh, xms, cps, threads):
+def cpu_baseline(p_gpu, mesh, xms, cps, threads, dev=None):
     """One Newton iteration of the oracle (complex-step assembly with OpenMP
-    over elements + SuperLU) on the same 64^2 state: the bounded sample."""
+    over elements + SuperLU) on the same 64^2 state: the bounded sample.
+    With `dev`, the device assembles and solves the same state (outside any
+    timed region) and the relative differences of r, K and du from the
+    oracle's are returned beside the baseline."""
     os.environ["OMP_NUM_THREADS"] = str(threads)
     import scipy.sparse.linalg as spla
 
@@ -91,12 +98,25 @@ def cpu_baseline(p_gpu, mesh, xms, cps, threads):
     t1 = time.perf_counter()
     du = spla.spsolve(K.tocsc(), -r)
     t2 = time.perf_counter()
-    return {"value": 1.0 / (t2 - t0), "unit": "Newton steps/s", "cores": threads,
+    base = {"value": 1.0 / (t2 - t0), "unit": "Newton steps/s", "cores": threads,
             "kind": "port",
             "sample": (f"1 Newton iteration at the 64x64 ALEVB bench state: oracle complex-step "
                        f"assembly {t1 - t0:.2f} s on {threads} OpenMP threads + scipy SuperLU "
                        f"solve {t2 - t1:.2f} s (1 thread); elem-assemblies/s "
                        f"{o.mesh.numel / (t1 - t0):.1f}")}
+    vs = None
+    if dev is not None:
+        dev.set_state(xms, cps)
+        r_g, K_g = dev.assemble(1e-3, 1e-3)
+        du_g, relres = dev.solve(-r_g)
+        dK = (K_g - K).tocsr()
+        vs = {"state": "bench initial state, t = dt = 1e-3",
+              "r_rel": float(np.linalg.norm(r_g - r) / np.linalg.norm(r)),
+              "K_rel_fro": float(np.sqrt((dK.multiply(dK)).sum()) /
+                                 np.sqrt((K.multiply(K)).sum())),
+              "du_rel": float(np.linalg.norm(du_g - du) / np.linalg.norm(du)),
+              "solve_relres": float(relres)}
+    return base, vs
 
 
 def batched_flops(patches):
@@ -528,7 +548,6 @@ def main():
         "max_solve_relres": max(relres),
         "factorizations": agg["n_factor"], "gmres_iters": agg["gmres_iters"],
         "band_solves": agg["n_band_solve"],
-        "rel_err_vs_fp64": 0.0,
     }
     # dominant kernel of the direct FP64 path: k_update_diag, one launch per
     # tile row of every factorization: the trailing update of step K (tb x tb
@@ -536,9 +555,18 @@ def main():
     # with the 64x64 diagonal LU of step K + 1 (2/3 * 64^3 FLOP) fused in;
     # the timed launch is the middle step, where all tb^2 tiles are in range.
     _, tb = dev_band_shape(dev)
-    kmid = nt // 2
-    n_tiles = min(tb, nt - 1 - kmid) ** 2
-    flops_step = n_tiles * 2.0 * TILE ** 3 + 2.0 / 3.0 * TILE ** 3
+    chains = st["factor_chains"]
+    if chains == 2:
+        # two-ended factorization: one launch advances both chains (the top
+        # part downward, the bottom part upward) by one tile step, each a full
+        # tb x tb window plus the next diagonal LU; the separator's tb steps
+        # follow as single-chain launches
+        a = (nt - tb) // 2
+        launches = max(a, nt - tb - a) + tb
+    else:
+        launches = nt
+    n_tiles = tb * tb
+    flops_step = chains * (n_tiles * 2.0 * TILE ** 3 + 2.0 / 3.0 * TILE ** 3)
     s_us = float(np.mean(step_us)) if step_us else float("nan")
     achieved = flops_step / (s_us * 1e-6) / 1e12
     # HBM bytes per k_update_diag launch, from two rocprofv3 --pmc passes over
@@ -548,22 +576,27 @@ def main():
     # `bench.py --copy-probe` (FETCH_SIZE 131088.375 KiB, WRITE_SIZE 262144 KiB
     # for 262144 KiB moved); profiles/k_update_diag_hbm_traffic_pmc.txt holds them.
     traffic = None
-    if STEP_FETCH_KIB is not None:
-        traffic = (STEP_FETCH_KIB * (262144.0 / 131088.375) + STEP_WRITE_KIB * 1.0) * 1024.0
+    counters = STEP_COUNTERS_KIB["k_update_diag2" if chains == 2 else "k_update_diag"]
+    if counters is not None:
+        traffic = (counters[0] * (262144.0 / 131088.375) + counters[1] * 1.0) * 1024.0
     s_bytes = TILE * TILE * 8
     res["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                        "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                        "traffic": traffic, "traffic_unit": "bytes per launch",
-                       "algorithmic_bytes": (2 * n_tiles + 2 * tb) * s_bytes,
-                       "kernel": "k_update_diag (trailing update of one factorization step on "
-                                 "the matrix cores, next 64x64 diagonal LU fused in)",
+                       "algorithmic_bytes": chains * (2 * n_tiles + 2 * tb) * s_bytes,
+                       "kernel": ("k_update_diag2 (one tile step of both chains of the two-ended "
+                                  "band LU: trailing updates on the matrix cores, each chain's "
+                                  "next 64x64 diagonal LU fused in)" if chains == 2 else
+                                  "k_update_diag (trailing update of one factorization step on "
+                                  "the matrix cores, next 64x64 diagonal LU fused in)"),
                        "flops_per_launch": flops_step, "avg_us": s_us,
-                       "launches_per_factorization": nt,
+                       "launches_per_factorization": launches,
                        "factorization_tflops": flops_fact / (agg["factor_us"] * 1e-6 /
                                                              max(1, agg["n_factor"])) / 1e12
                        if agg["factor_us"] > 0 else None}
     if not args.no_cpu_baseline:
-        res["cpu_baseline"] = cpu_baseline(p, mesh, xms, cps, args.cpu_threads)
+        res["cpu_baseline"], res["vs_oracle"] = cpu_baseline(p, mesh, xms, cps, args.cpu_threads,
+                                                             dev)
     else:
         res["cpu_baseline"] = None
     dev.close()

@@ -120,6 +120,9 @@ typedef struct {
                                       zero) replaced by +-eps max|K|: static pivoting,
                                       switched on for a context once a factorization
                                       without it gave a failed solve */
+  int32_t factor_chains;           /* 2: two-ended band LU (top part downward and bottom
+                                      part upward at once, separator last); 1: one chain */
+  int32_t separator_rows;          /* 64-row tile rows of the separator (0 with one chain) */
 } mafem_stats;
 
 /* Builds the device context: uploads the tables, builds the block-sparse

@@ -57,7 +57,8 @@ void free_all(Ctx &c) {
                   c.d_uinv, c.d_piv,  c.d_V,       c.d_Z,       c.d_h,      c.d_coef,
                   c.d_mdot_part, c.d_t2, c.d_presc_idx, c.d_presc_val, c.d_s1, c.d_s2,
                   c.d_sweep_err, c.d_pf_el, c.d_pf_slot, c.d_pf_rel, c.d_pf_out,
-                  c.d_obel, c.d_crnr, c.d_out, c.d_amax};
+                  c.d_obel, c.d_crnr, c.d_out, c.d_amax, c.d_tw_z, c.d_tw_br,
+                  c.d_tw_cr, c.d_tw_dr, c.d_tw_s1, c.d_tw_s2};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c.h_red) (void)hipHostFree(c.h_red);
@@ -381,7 +382,17 @@ void build(Ctx &c, const mafem_mesh_desc *m, const mafem_params *p) {
   if (c.tb > c.nt - 1) c.tb = (int)std::max<int64_t>(1, c.nt - 1);
   if (c.tb + 1 > SOLVE_RING)
     throw ArgError{"patch too wide for the band solver (num1el <= 262)", MAFEM_EUNSUPPORTED};
-  c.band_bytes = c.nt * (2 * c.tb + 1) * (int64_t)TILE * TILE * (int64_t)sizeof(double);
+  // two-ended factorization (T down, B up, separator last) once the chain of
+  // tile rows is long enough for halving it to pay; MAFEM_TWO_ENDED=0 keeps one
+  // chain
+  {
+    const char *e = std::getenv("MAFEM_TWO_ENDED");
+    if (c.nt >= 8 * (int64_t)c.tb && !(e && std::atoi(e) == 0)) {
+      c.tw_a = (c.nt - c.tb) / 2;
+      c.tw_nb = c.nt - c.tb - c.tw_a;
+    }
+  }
+  c.band_bytes = c.band_rows() * (2 * c.tb + 1) * (int64_t)TILE * TILE * (int64_t)sizeof(double);
 
   // device allocations and uploads
   MAFEM_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
@@ -420,9 +431,19 @@ void build(Ctx &c, const mafem_mesh_desc *m, const mafem_params *p) {
   MAFEM_HIP(hipHostMalloc((void **)&c.h_red, sizeof(double) * 16, hipHostMallocDefault));
   c.d_eq = dupload(c, eq_ext);
   c.opts = mafem_solver_opts{MAFEM_SOLVER_DIRECT, 64, 30, 200, 20, 3, 1e-13};
-  c.d_piv = dalloc<int32_t>(c, c.nt * TILE);
+  c.d_piv = dalloc<int32_t>(c, c.band_rows() * TILE);
   // sweeps over several workgroups once the chain of tile rows is long
-  c.sweep_groups = (int)std::min<int64_t>(16, std::max<int64_t>(1, c.nt / 32));
+  const int64_t chain = c.tw_a > 0 ? std::min(c.tw_a, c.tw_nb) : c.nt;
+  c.sweep_groups = (int)std::min<int64_t>(16, std::max<int64_t>(1, chain / 32));
+  if (c.tw_a > 0) {
+    const int64_t nr = (c.tw_nb + c.tb) * TILE;
+    c.d_tw_z = dalloc<double>(c, nv);
+    c.d_tw_br = dalloc<double>(c, nr);
+    c.d_tw_cr = dalloc<double>(c, nr);
+    c.d_tw_dr = dalloc<double>(c, nr);
+    c.d_tw_s1 = dalloc<double>(c, nr);
+    c.d_tw_s2 = dalloc<double>(c, nr);
+  }
   if (const char *e = std::getenv("MAFEM_SWEEP_GROUPS")) {
     const int g = std::atoi(e);
     if (g >= 1 && g <= 64) c.sweep_groups = g;
@@ -545,8 +566,9 @@ void tune_sweeps(Ctx &c, const double *d_b) {
   if (c.sweep_groups <= 1 || std::getenv("MAFEM_SWEEP_GROUPS")) return;
   float best = 1e30f;
   int best_w = c.sweep_groups;
+  const int64_t chain = c.tw_a > 0 ? std::min(c.tw_a, c.tw_nb) : c.nt;
   for (int w : {2, 4, 8, 16}) {
-    if (w > c.nt / 4) break;
+    if (w > chain / 4) break;
     c.sweep_groups = w;
     band_solve(c, d_b, c.d_t);  // warm
     MAFEM_HIP(hipEventRecord(c.ev[6], c.stream));
@@ -815,6 +837,9 @@ int mafem_solve(mafem_ctx *ctx, const double *rhs, double *du, double *relres) {
     c.stats.solve_us = 1e3 * elapsed(c, 3, 4);
     c.stats.n_factor = c.n_factor - nf0;
     c.stats.gmres_iters = c.n_gmres - ng0;
+    c.stats.sweep_groups = c.sweep_groups;
+    c.stats.factor_chains = c.tw_a > 0 ? 2 : 1;
+    c.stats.separator_rows = c.tw_a > 0 ? c.tb : 0;
     return MAFEM_OK;
   });
 }
@@ -870,6 +895,8 @@ int mafem_time_step(mafem_ctx *ctx, double time, double dt, int32_t max_iter, in
     if (n_step > 0) st.step_kernel_us /= n_step;
     st.sweep_groups = c.sweep_groups;
     st.pivot_perturbations = read_pert(c) - np0;
+    st.factor_chains = c.tw_a > 0 ? 2 : 1;
+    st.separator_rows = c.tw_a > 0 ? c.tb : 0;
     c.stats = st;
     if (iters) *iters = it;
     if (!(eps < c.enr)) {

@@ -69,6 +69,107 @@ __global__ void k_band_pad(int64_t nx, int64_t npad, int tb, S *__restrict__ ban
   tile_ptr(band, tb, R / T, R / T)[(R % T) * T + (R % T)] = S(1);
 }
 
+// ------------------------------------------- two-ended layout (Ctx::tw_a > 0)
+//
+// Scalar row R belongs to the top band BT while R < ns = (a + w) TILE (T and
+// the separator S, original order); the bottom band BR holds B and S in
+// reversed order, R -> N - 1 - R (N = nt TILE).  An entry between two rows
+// of S goes to BT only: BR's separator block starts at zero and collects B's
+// Schur complement.  T and B never couple (they are more than tb tiles apart).
+template <class S>
+__global__ __launch_bounds__(256) void k_bsr_to_band_tw(int64_t nnzb, int ndf, int tb, int64_t ns,
+                                                        int64_t N, const int32_t *__restrict__ brow,
+                                                        const int32_t *__restrict__ colidx,
+                                                        const double *__restrict__ bsr,
+                                                        S *__restrict__ bt, S *__restrict__ br) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int bb = ndf * ndf;
+  if (t >= nnzb * bb) return;
+  const int64_t blk = t / bb;
+  const int rc = (int)(t % bb), r = rc / ndf, c = rc % ndf;
+  int64_t R = (int64_t)brow[blk] * ndf + r, C = (int64_t)colidx[blk] * ndf + c;
+  S *band = bt;
+  if (R >= ns || C >= ns) {
+    R = N - 1 - R;
+    C = N - 1 - C;
+    band = br;
+  }
+  tile_ptr(band, tb, R / T, C / T)[(R % T) * T + (C % T)] = (S)bsr[t];
+}
+
+// identity on the padding rows [nx, N), which lie in B (reversed: BR's first rows)
+template <class S>
+__global__ void k_band_pad_tw(int64_t nx, int64_t N, int tb, S *__restrict__ br) {
+  const int64_t R = nx + (int64_t)blockIdx.x * 64 + threadIdx.x;
+  if (R >= N) return;
+  const int64_t Rr = N - 1 - R;
+  tile_ptr(br, tb, Rr / T, Rr / T)[(Rr % T) * T + (Rr % T)] = S(1);
+}
+
+// BT's separator tile (I, J) += BR's tile (nt-1-I, nt-1-J) read in reversed
+// element order (w x w tiles, one block each)
+template <class S>
+__global__ __launch_bounds__(256) void k_sep_add(int64_t a, int w, int tb, int64_t nt,
+                                                 S *__restrict__ bt, const S *__restrict__ br) {
+  const int64_t I = a + blockIdx.x / w, J = a + blockIdx.x % w;
+  S *d = tile_ptr(bt, tb, I, J);
+  const S *s = tile_ptr(br, tb, nt - 1 - I, nt - 1 - J);
+  for (int e = threadIdx.x; e < T * T; e += 256) d[e] += s[T * T - 1 - e];
+}
+
+// out_I -= Σ_K M_IK v_K over the band tiles (I, K), K in [max(klo, I - tb),
+// min(khi, I + tb + 1)), for rows I = I0 + blockIdx.x; tiles in increasing K.
+// vrev / orev > 0 address v / out in reversed order (element k of tile K at
+// vrev - 1 - (K TILE + k)).  ATOMIC: v is a multi-workgroup sweep output.
+template <class S, bool ATOMIC>
+__global__ __launch_bounds__(256) void k_rows_sub(int tb, const S *__restrict__ band, int64_t I0,
+                                                  int64_t klo, int64_t khi,
+                                                  const double *__restrict__ v, int64_t vrev,
+                                                  double *__restrict__ out, int64_t orev) {
+  __shared__ double sv[T];
+  const int64_t I = I0 + blockIdx.x;
+  const int i = threadIdx.x / 4, q = threadIdx.x % 4;
+  const int64_t K0 = I - tb > klo ? I - tb : klo;
+  const int64_t K1 = I + tb + 1 < khi ? I + tb + 1 : khi;
+  double s = 0.0;
+  for (int64_t K = K0; K < K1; ++K) {
+    __syncthreads();
+    if (threadIdx.x < T) {
+      int64_t k = K * T + threadIdx.x;
+      if (vrev > 0) k = vrev - 1 - k;
+      sv[threadIdx.x] = ATOMIC ? __longlong_as_double((long long)__hip_atomic_load(
+                                     reinterpret_cast<const unsigned long long *>(v) + k,
+                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                               : v[k];
+    }
+    __syncthreads();
+    const S *m = tile_ptr(band, tb, I, K) + i * T + q * 16;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) s = fma((double)m[k], sv[q * 16 + k], s);
+  }
+  s += __shfl_xor(s, 1, 64);
+  s += __shfl_xor(s, 2, 64);
+  if (q == 0) {
+    int64_t k = I * T + i;
+    if (orev > 0) k = orev - 1 - k;
+    out[k] -= s;
+  }
+}
+
+// dst[k] = src[N - 1 - k] for k in [k0, k0 + n); ATOMIC: src is a sweep output
+template <bool ATOMIC>
+__global__ __launch_bounds__(256) void k_rev_copy(int64_t k0, int64_t n, int64_t N,
+                                                  const double *__restrict__ src,
+                                                  double *__restrict__ dst) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  const int64_t k = k0 + t;
+  dst[k] = ATOMIC ? __longlong_as_double((long long)__hip_atomic_load(
+                        reinterpret_cast<const unsigned long long *>(src) + (N - 1 - k),
+                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                  : src[N - 1 - k];
+}
+
 // ------------------------------------------------------------ factorization
 
 // Maximum over the wavefront of a 32-bit key, delivered as a scalar: DPP
@@ -500,6 +601,45 @@ __global__ __launch_bounds__(256) void k_update_diag(int64_t K, int64_t nt, int 
   S(*u12)[T] = reinterpret_cast<S(*)[T]>(lds + T * (T + 1));
   S(*l11)[16] = reinterpret_cast<S(*)[16]>(lds + T * (T + 1) + 16 * T);
   diag_core<S, PERT>(K + 1, tile_ptr(band, tb, K + 1, K + 1), piv, st, u12, l11, amax);
+}
+
+// Both chains of the two-ended factorization in one launch (blocks of chain 1
+// follow those of chain 0).  A chain with fuse = 0 takes the plain trailing
+// update (its next diagonal tile is not complete yet); nt = 0 leaves it out.
+struct FacChain {
+  void *band;
+  int32_t *piv;
+  int64_t K, nt;
+  int32_t fuse;
+};
+
+template <class S>
+__global__ __launch_bounds__(64) void k_panel2(int tb, FacChain c0, FacChain c1) {
+  const int nb0 = 3 * tb;
+  const bool second = (int)blockIdx.x >= nb0;
+  const FacChain &ch = second ? c1 : c0;
+  if (ch.nt == 0) return;
+  panel_body<S>((int64_t)blockIdx.x - (second ? nb0 : 0), ch.K, ch.nt, tb, (S *)ch.band, ch.piv);
+}
+
+template <class S, bool PERT>
+__global__ __launch_bounds__(256) void k_update_diag2(int tb, FacChain c0, FacChain c1,
+                                                      unsigned long long *__restrict__ amax) {
+  __shared__ __align__(16) S lds[MM_LDS];
+  const int nb0 = tb * tb;
+  const bool second = (int)blockIdx.x >= nb0;
+  const FacChain &ch = second ? c1 : c0;
+  if (ch.nt == 0) return;
+  const int64_t bid = (int64_t)blockIdx.x - (second ? nb0 : 0);
+  S *band = (S *)ch.band;
+  const bool fused = bid == 0 && ch.fuse;
+  update_body<S>(bid, ch.K, ch.nt, tb, band, lds, fused);
+  if (!fused) return;
+  __syncthreads();
+  S(*st)[T + 1] = reinterpret_cast<S(*)[T + 1]>(lds);
+  S(*u12)[T] = reinterpret_cast<S(*)[T]>(lds + T * (T + 1));
+  S(*l11)[16] = reinterpret_cast<S(*)[16]>(lds + T * (T + 1) + 16 * T);
+  diag_core<S, PERT>(ch.K + 1, tile_ptr(band, tb, ch.K + 1, ch.K + 1), ch.piv, st, u12, l11, amax);
 }
 
 // ------------------------------------------------- post-factorization pass
@@ -986,6 +1126,38 @@ __global__ __launch_bounds__(SW) void k_bwd_sweep(int64_t nt, int tb, const S *_
   bwd_sweep_body<S>(nt, tb, band, x);
 }
 
+// The two chains of the two-ended solve (T in BT, B in BR) in one launch:
+// chain = blockIdx.x / W.  Single workgroup (W = 1): in place on y.
+// Multi-workgroup: c in, y out (sentinel-filled, atomic access only).
+struct SweepChain {
+  int64_t nt;
+  const void *band;
+  const double *c;
+  double *y;
+};
+
+template <class S, int DIR>
+__global__ __launch_bounds__(SW) void k_sweep2(int tb, SweepChain a0, SweepChain a1) {
+  const SweepChain &a = blockIdx.x == 0 ? a0 : a1;
+  if (DIR == 0)
+    fwd_sweep_body<S>(a.nt, tb, (const S *)a.band, a.y);
+  else
+    bwd_sweep_body<S>(a.nt, tb, (const S *)a.band, a.y);
+}
+
+template <class S, int DIR>
+__global__ __launch_bounds__(SW) void k_sweep2_mw(int tb, int W, SweepChain a0, SweepChain a1,
+                                                  int *__restrict__ err) {
+  const int chain = (int)blockIdx.x / W, w = (int)blockIdx.x % W;
+  const SweepChain &a = chain == 0 ? a0 : a1;
+  if (DIR == 0)
+    fwd_sweep_mw_body<S>(w, W, a.nt, tb, (const S *)a.band, a.c,
+                         reinterpret_cast<unsigned long long *>(a.y), err);
+  else
+    bwd_sweep_mw_body<S>(w, W, a.nt, tb, (const S *)a.band, a.c,
+                         reinterpret_cast<unsigned long long *>(a.y), err);
+}
+
 // ------------------------------------------------------------ batched forms
 //
 // Patch p takes part in a factorization launch when on[p] && fac[p], and in
@@ -1228,56 +1400,224 @@ static inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 template <class S>
 static void fill_t(Ctx &c, S *band) {
-  const int64_t nband = c.nt * (2 * c.tb + 1) * (int64_t)T * T;
+  const int64_t nband = c.band_rows() * (2 * c.tb + 1) * (int64_t)T * T;
   // grid-stride over a capped grid: a dispatch counts its work-items in 32
   // bits, and a large patch's band (256x256 ALEVB: 4.5e9 entries) exceeds that
   const unsigned zb = (unsigned)std::min<int64_t>(nblk(nband), 1 << 20);
   hipLaunchKernelGGL(k_zero_t<S>, dim3(zb), dim3(256), 0, c.stream, nband, band);
   const int64_t nb = c.nnzb * c.ndf * c.ndf;
+  const int64_t npad = c.nt * T;
+  if (c.tw_a > 0) {
+    const int64_t rows_t = c.tw_a + c.tb;  // BT: T and the separator
+    S *br = band + rows_t * (2 * c.tb + 1) * (int64_t)T * T;
+    hipLaunchKernelGGL(k_bsr_to_band_tw<S>, dim3(nblk(nb)), dim3(256), 0, c.stream, c.nnzb, c.ndf,
+                       c.tb, rows_t * T, npad, c.d_brow, c.d_colidx, c.d_bsr, band, br);
+    if (npad > c.nx)
+      hipLaunchKernelGGL(k_band_pad_tw<S>, dim3((unsigned)((npad - c.nx + 63) / 64)), dim3(64), 0,
+                         c.stream, c.nx, npad, c.tb, br);
+    MAFEM_HIP(hipGetLastError());
+    return;
+  }
   hipLaunchKernelGGL(k_bsr_to_band<S>, dim3(nblk(nb)), dim3(256), 0, c.stream, c.nnzb, c.ndf, c.tb,
                      c.d_brow, c.d_colidx, c.d_bsr, band);
-  const int64_t npad = c.nt * T;
   if (npad > c.nx)
     hipLaunchKernelGGL(k_band_pad<S>, dim3((unsigned)((npad - c.nx + 63) / 64)), dim3(64), 0,
                        c.stream, c.nx, npad, c.tb, band);
   MAFEM_HIP(hipGetLastError());
 }
 
+// Steps k0 .. k1 - 1 of the band LU of a band with nt tile rows on stream s:
+// the diagonal LU of step k0 first (diag_first), then per step the panels
+// and the trailing update with the next diagonal LU fused in (k_update_diag),
+// except after the last step when !fuse_last (its next diagonal tile is not
+// complete yet).  When `timed`, one launch in the middle is bracketed by
+// ev[10] / ev[11].
+template <class S, bool PERT>
+static void factor_chain(Ctx &c, hipStream_t s, S *band, int32_t *piv, int64_t nt, int64_t k0,
+                         int64_t k1, bool diag_first, bool fuse_last, bool timed) {
+  if (diag_first)
+    hipLaunchKernelGGL((k_diag<S, PERT>), dim3(1), dim3(64), 0, s, k0, c.tb, band, piv, c.d_amax);
+  const int64_t timed_K = timed ? (k0 + k1) / 2 : -1;
+  for (int64_t K = k0; K < k1; ++K) {
+    hipLaunchKernelGGL(k_panel<S>, dim3(3 * c.tb), dim3(64), 0, s, K, nt, c.tb, band, piv);
+    if (K + 1 >= nt) break;  // no trailing tiles after the last step
+    if (K + 1 < k1 || fuse_last) {
+      const bool tk = (K == timed_K);
+      if (tk) MAFEM_HIP(hipEventRecord(c.ev[10], s));
+      hipLaunchKernelGGL((k_update_diag<S, PERT>), dim3(c.tb * c.tb), dim3(256), 0, s, K, nt, c.tb,
+                         band, piv, c.d_amax);
+      if (tk) {
+        MAFEM_HIP(hipEventRecord(c.ev[11], s));
+        c.step_timed = true;
+      }
+    } else {
+      hipLaunchKernelGGL(k_update<S>, dim3(c.tb * c.tb), dim3(256), 0, s, K, nt, c.tb, band);
+    }
+  }
+}
+
 template <class S, bool PERT>
 static void factor_run(Ctx &c, S *band, S *li, S *ui) {
-  // the diagonal LU of step K + 1 runs inside step K's trailing update
-  // (k_update_diag); one launch in the middle of the chain is timed
-  const int64_t timed_K = c.nt / 2;
   if (PERT) {  // max |K| of this tangent for the static-pivoting threshold
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c.stream, c.d_amax);
     const int64_t nb = c.nnzb * c.ndf * c.ndf;
     hipLaunchKernelGGL(k_absmax<double>, dim3(1024), dim3(256), 0, c.stream, nb,
                        (const double *)c.d_bsr, c.d_amax);
   }
+  if (c.tw_a == 0) {
+    factor_chain<S, PERT>(c, c.stream, band, c.d_piv, c.nt, 0, c.nt, true, true, true);
+    hipLaunchKernelGGL(k_tile_inv<S>, dim3((unsigned)c.nt), dim3(64), 0, c.stream, c.tb, band, li, ui);
+    hipLaunchKernelGGL(k_fold<S>, dim3((unsigned)(c.nt * (2 * c.tb + 1))), dim3(256), 0, c.stream,
+                       c.nt, c.tb, band, li, ui);
+    MAFEM_HIP(hipGetLastError());
+    return;
+  }
+  // two-ended: step k of T (downward) and of B (upward, reversed band) in the
+  // same launches, then the separator, which has both Schur complements added
+  const int64_t a = c.tw_a, nb = c.tw_nb, w = c.tb, rt = a + w, rb = nb + w;
+  const int64_t tsz = (int64_t)T * T;
+  S *br = band + rt * (2 * c.tb + 1) * tsz;
+  int32_t *pr = c.d_piv + rt * T;
   hipLaunchKernelGGL((k_diag<S, PERT>), dim3(1), dim3(64), 0, c.stream, (int64_t)0, c.tb, band,
                      c.d_piv, c.d_amax);
-  for (int64_t K = 0; K < c.nt; ++K) {
-    hipLaunchKernelGGL(k_panel<S>, dim3(3 * c.tb), dim3(64), 0, c.stream, K, c.nt, c.tb, band,
-                       c.d_piv);
-    if (K + 1 >= c.nt) break;  // no trailing tiles after the last step
-    const bool timed = (K == timed_K);
-    if (timed) MAFEM_HIP(hipEventRecord(c.ev[10], c.stream));
-    hipLaunchKernelGGL((k_update_diag<S, PERT>), dim3(c.tb * c.tb), dim3(256), 0, c.stream, K,
-                       c.nt, c.tb, band, c.d_piv, c.d_amax);
-    if (timed) {
+  hipLaunchKernelGGL((k_diag<S, PERT>), dim3(1), dim3(64), 0, c.stream, (int64_t)0, c.tb, br, pr,
+                     c.d_amax);
+  const int64_t steps = std::max(a, nb), timed_k = std::min(a, nb) / 2;
+  for (int64_t k = 0; k < steps; ++k) {
+    // a chain past its last step drops out (nt = 0); each chain's last step
+    // leaves its separator diagonal tile unfactored (fuse = 0)
+    const FacChain ct{band, c.d_piv, k, k < a ? rt : 0, k + 1 < a ? 1 : 0};
+    const FacChain cb{br, pr, k, k < nb ? rb : 0, k + 1 < nb ? 1 : 0};
+    hipLaunchKernelGGL(k_panel2<S>, dim3(6 * c.tb), dim3(64), 0, c.stream, c.tb, ct, cb);
+    const bool tk = (k == timed_k);
+    if (tk) MAFEM_HIP(hipEventRecord(c.ev[10], c.stream));
+    hipLaunchKernelGGL((k_update_diag2<S, PERT>), dim3(2 * c.tb * c.tb), dim3(256), 0, c.stream,
+                       c.tb, ct, cb, c.d_amax);
+    if (tk) {
       MAFEM_HIP(hipEventRecord(c.ev[11], c.stream));
       c.step_timed = true;
     }
   }
-  hipLaunchKernelGGL(k_tile_inv<S>, dim3((unsigned)c.nt), dim3(64), 0, c.stream, c.tb, band, li, ui);
-  hipLaunchKernelGGL(k_fold<S>, dim3((unsigned)(c.nt * (2 * c.tb + 1))), dim3(256), 0, c.stream,
-                     c.nt, c.tb, band, li, ui);
+  hipLaunchKernelGGL(k_sep_add<S>, dim3((unsigned)(w * w)), dim3(256), 0, c.stream, a, (int)w, c.tb,
+                     c.nt, band, (const S *)br);
+  factor_chain<S, PERT>(c, c.stream, band, c.d_piv, rt, a, rt, true, true, false);
+  // inverses of the diagonal tiles; folds: every tile row of BT, B's rows of
+  // BR (its separator rows keep their raw L tiles for the solve)
+  hipLaunchKernelGGL(k_tile_inv<S>, dim3((unsigned)rt), dim3(64), 0, c.stream, c.tb, band, li, ui);
+  hipLaunchKernelGGL(k_tile_inv<S>, dim3((unsigned)nb), dim3(64), 0, c.stream, c.tb, (const S *)br,
+                     li + rt * tsz, ui + rt * tsz);
+  hipLaunchKernelGGL(k_fold<S>, dim3((unsigned)(rt * (2 * c.tb + 1))), dim3(256), 0, c.stream, rt,
+                     c.tb, band, li, ui);
+  hipLaunchKernelGGL(k_fold<S>, dim3((unsigned)(nb * (2 * c.tb + 1))), dim3(256), 0, c.stream, rb,
+                     c.tb, br, li + rt * tsz, ui + rt * tsz);
+  MAFEM_HIP(hipGetLastError());
+}
+
+// Two-ended solve (Ctx::tw_a > 0).  Forward: c = L_II^-1 P b on the rows of T
+// and (reversed) of B, both forward sweeps at once; the separator's
+// right-hand side loses B's part (BR's raw separator L tiles against B's
+// solution, reversed) before its own L_II^-1 P, then T's part (BT's folded
+// tiles) and its own sweep.  Backward: U_II^-1 everywhere, the separator's
+// backward sweep, its terms in the last tb rows of T and of B, then both
+// backward sweeps at once.
+template <class S>
+static void solve_tw(Ctx &c, const S *band, const S *li, const S *ui, const double *d_rhs,
+                     double *d_x) {
+  const int64_t a = c.tw_a, nb = c.tw_nb, w = c.tb, rt = a + w, tsz = (int64_t)T * T;
+  const int64_t N = c.nt * T, ns = rt * T;
+  const int tb = c.tb;
+  const S *bt = band, *br = band + rt * (2 * tb + 1) * tsz;
+  const S *sep = bt + a * (2 * tb + 1) * tsz;  // BT from the separator's first row
+  const S *liR = li + rt * tsz, *uiR = ui + rt * tsz;
+  const int32_t *pivR = c.d_piv + rt * T;
+  const hipStream_t s = c.stream;
+  const bool mw = c.sweep_groups > 1;
+  const int W = c.sweep_groups;
+  hipLaunchKernelGGL(k_rev_copy<false>, dim3(nblk(nb * T)), dim3(256), 0, s, (int64_t)0, nb * T, N,
+                     d_rhs, c.d_tw_br);
+  hipLaunchKernelGGL(k_diag_apply<S>, dim3((unsigned)a), dim3(256), 0, s, li, c.d_piv, d_rhs, c.d_t2);
+  hipLaunchKernelGGL(k_diag_apply<S>, dim3((unsigned)nb), dim3(256), 0, s, liR, pivR,
+                     (const double *)c.d_tw_br, c.d_tw_cr);
+  if (mw) {
+    hipLaunchKernelGGL(k_fill_sentinel, dim3(nblk(a * T)), dim3(256), 0, s, a * T,
+                       (unsigned long long *)c.d_s1);
+    hipLaunchKernelGGL(k_fill_sentinel, dim3(nblk(nb * T)), dim3(256), 0, s, nb * T,
+                       (unsigned long long *)c.d_tw_s1);
+    hipLaunchKernelGGL((k_sweep2_mw<S, 0>), dim3(2 * W), dim3(SW), 0, s, tb, W,
+                       SweepChain{a, bt, c.d_t2, c.d_s1}, SweepChain{nb, br, c.d_tw_cr, c.d_tw_s1},
+                       c.d_sweep_err);
+  } else {
+    hipLaunchKernelGGL((k_sweep2<S, 0>), dim3(2), dim3(SW), 0, s, tb,
+                       SweepChain{a, bt, nullptr, c.d_t2}, SweepChain{nb, br, nullptr, c.d_tw_cr});
+  }
+  const double *yT = mw ? c.d_s1 : c.d_t2, *yB = mw ? c.d_tw_s1 : c.d_tw_cr;
+  MAFEM_HIP(hipMemcpyAsync(c.d_tw_z + a * T, d_rhs + a * T, sizeof(double) * w * T,
+                           hipMemcpyDeviceToDevice, s));
+  if (mw)
+    hipLaunchKernelGGL((k_rows_sub<S, true>), dim3((unsigned)w), dim3(256), 0, s, tb, br, nb,
+                       (int64_t)0, nb, yB, (int64_t)0, c.d_tw_z, N);
+  else
+    hipLaunchKernelGGL((k_rows_sub<S, false>), dim3((unsigned)w), dim3(256), 0, s, tb, br, nb,
+                       (int64_t)0, nb, yB, (int64_t)0, c.d_tw_z, N);
+  hipLaunchKernelGGL(k_diag_apply<S>, dim3((unsigned)w), dim3(256), 0, s, li + a * tsz,
+                     c.d_piv + a * T, (const double *)c.d_tw_z + a * T, c.d_t2 + a * T);
+  if (mw)
+    hipLaunchKernelGGL((k_rows_sub<S, true>), dim3((unsigned)w), dim3(256), 0, s, tb, bt, a,
+                       (int64_t)0, a, yT, (int64_t)0, c.d_t2, (int64_t)0);
+  else
+    hipLaunchKernelGGL((k_rows_sub<S, false>), dim3((unsigned)w), dim3(256), 0, s, tb, bt, a,
+                       (int64_t)0, a, yT, (int64_t)0, c.d_t2, (int64_t)0);
+  hipLaunchKernelGGL(k_fwd_sweep<S>, dim3(1), dim3(SW), 0, s, w, tb, sep, c.d_t2 + a * T);
+  // d = U_II^-1 y (in place per tile row where y is plain)
+  if (mw) {
+    hipLaunchKernelGGL(k_diag_apply_atomic<S>, dim3((unsigned)a), dim3(256), 0, s, ui, yT, c.d_t2);
+    hipLaunchKernelGGL(k_diag_apply_atomic<S>, dim3((unsigned)nb), dim3(256), 0, s, uiR, yB,
+                       c.d_tw_dr);
+  } else {
+    hipLaunchKernelGGL(k_diag_apply<S>, dim3((unsigned)a), dim3(256), 0, s, ui,
+                       (const int32_t *)nullptr, yT, c.d_t2);
+    hipLaunchKernelGGL(k_diag_apply<S>, dim3((unsigned)nb), dim3(256), 0, s, uiR,
+                       (const int32_t *)nullptr, yB, c.d_tw_dr);
+  }
+  hipLaunchKernelGGL(k_diag_apply<S>, dim3((unsigned)w), dim3(256), 0, s, ui + a * tsz,
+                     (const int32_t *)nullptr, (const double *)c.d_t2 + a * T, c.d_t2 + a * T);
+  hipLaunchKernelGGL(k_bwd_sweep<S>, dim3(1), dim3(SW), 0, s, w, tb, sep, c.d_t2 + a * T);
+  const int64_t r0 = a - tb > 0 ? a - tb : 0, q0 = nb - tb > 0 ? nb - tb : 0;
+  hipLaunchKernelGGL((k_rows_sub<S, false>), dim3((unsigned)(a - r0)), dim3(256), 0, s, tb, bt, r0,
+                     a, rt, (const double *)c.d_t2, (int64_t)0, c.d_t2, (int64_t)0);
+  hipLaunchKernelGGL((k_rows_sub<S, false>), dim3((unsigned)(nb - q0)), dim3(256), 0, s, tb, br, q0,
+                     nb, nb + w, (const double *)c.d_t2, N, c.d_tw_dr, (int64_t)0);
+  if (mw) {
+    hipLaunchKernelGGL(k_fill_sentinel, dim3(nblk(a * T)), dim3(256), 0, s, a * T,
+                       (unsigned long long *)c.d_s2);
+    hipLaunchKernelGGL(k_fill_sentinel, dim3(nblk(nb * T)), dim3(256), 0, s, nb * T,
+                       (unsigned long long *)c.d_tw_s2);
+    hipLaunchKernelGGL((k_sweep2_mw<S, 1>), dim3(2 * W), dim3(SW), 0, s, tb, W,
+                       SweepChain{a, bt, c.d_t2, c.d_s2}, SweepChain{nb, br, c.d_tw_dr, c.d_tw_s2},
+                       c.d_sweep_err);
+    hipLaunchKernelGGL(k_collect, dim3(nblk(a * T)), dim3(256), 0, s, a * T,
+                       (const unsigned long long *)c.d_s2, d_x);
+    hipLaunchKernelGGL(k_rev_copy<true>, dim3(nblk(nb * T)), dim3(256), 0, s, ns, nb * T, N,
+                       (const double *)c.d_tw_s2, d_x);
+  } else {
+    hipLaunchKernelGGL((k_sweep2<S, 1>), dim3(2), dim3(SW), 0, s, tb,
+                       SweepChain{a, bt, nullptr, c.d_t2}, SweepChain{nb, br, nullptr, c.d_tw_dr});
+    MAFEM_HIP(hipMemcpyAsync(d_x, c.d_t2, sizeof(double) * a * T, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_rev_copy<false>, dim3(nblk(nb * T)), dim3(256), 0, s, ns, nb * T, N,
+                       (const double *)c.d_tw_dr, d_x);
+  }
+  MAFEM_HIP(hipMemcpyAsync(d_x + a * T, c.d_t2 + a * T, sizeof(double) * w * T,
+                           hipMemcpyDeviceToDevice, s));
   MAFEM_HIP(hipGetLastError());
 }
 
 template <class S>
 static void solve_t(Ctx &c, const S *band, const S *li, const S *ui, const double *d_rhs,
                     double *d_x) {
+  if (c.tw_a > 0) {
+    solve_tw<S>(c, band, li, ui, d_rhs, d_x);
+    return;
+  }
   if (c.sweep_groups > 1) {
     // c = L_II^-1 P b -> t2; y (sentinel-filled s1) = forward sweep; d = U_II^-1 y -> t2;
     // x (sentinel-filled s2) = backward sweep, collected into d_x
@@ -1312,7 +1652,8 @@ static void solve_t(Ctx &c, const S *band, const S *li, const S *ui, const doubl
 
 // Allocates the band storage of the requested precision on first use.
 static void ensure_band(Ctx &c, int prec) {
-  const int64_t ntile = c.nt * (2 * c.tb + 1) * (int64_t)T * T, ninv = c.nt * (int64_t)T * T;
+  const int64_t ntile = c.band_rows() * (2 * c.tb + 1) * (int64_t)T * T;
+  const int64_t ninv = c.band_rows() * (int64_t)T * T;
   const size_t es = (prec == 32) ? sizeof(float) : sizeof(double);
   if (c.band_prec == prec && c.d_band) return;
   std::lock_guard<std::mutex> no_capture(capture_mutex());  // allocation: not during a capture

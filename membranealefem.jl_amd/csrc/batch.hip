@@ -544,7 +544,12 @@ int mafem_batch_time_step(mafem_batch *bp, const double *time, const double *dt,
     MAFEM_HIP(hipSetDevice(b.dev));
     BatchPlan &B = b.B;
     const int P = B.P;
-    for (mafem_ctx *m : b.members) MAFEM_HIP(hipStreamSynchronize(m->c.stream));
+    for (mafem_ctx *m : b.members) {
+      MAFEM_HIP(hipStreamSynchronize(m->c.stream));
+      // the batch factors into the member's band in its own (one-chain) layout:
+      // a factorization the context kept for its own Krylov solves is gone
+      m->c.have_factor = false;
+    }
     if (plan_stale(b)) build_plan(b);
     ensure_hist(b, max_iter);
     BatchFlags &f = B.f;

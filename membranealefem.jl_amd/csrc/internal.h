@@ -141,6 +141,23 @@ struct Ctx {
   unsigned long long *d_amax = nullptr;
   bool perturb_pivots = false;   // set once a factorization without it failed
 
+  // Two-ended band LU of the single-context solve (tw_a > 0).  The tile rows
+  // split into a top part T = [0, tw_a), a separator S of tb rows and a bottom
+  // part B of tw_nb rows; T and B do not couple.  T is factored downward in
+  // the band BT (rows T and S, in original order), B upward in the band BR
+  // (rows B then S, both in reversed order: tile row I' = nt - 1 - I, element
+  // (i, j) -> (TILE-1-i, TILE-1-j)), step k of both chains in the same
+  // launches; both add their Schur complements into S, which BT factors last.
+  // The solve runs both chains' sweeps in the same launches too.  Storage:
+  // BT at tile row 0 of d_band / d_linv / d_uinv / d_piv, BR at tile row
+  // tw_a + tb.  tw_a = 0: one band factored top to bottom.
+  int64_t tw_a = 0, tw_nb = 0;
+  double *d_tw_z = nullptr;      // separator right-hand side (original order, nt TILE)
+  double *d_tw_br = nullptr;     // right-hand side in reversed order ((tw_nb + tb) TILE)
+  double *d_tw_cr = nullptr, *d_tw_dr = nullptr;  // BR forward input / backward input (tw_nb TILE)
+  double *d_tw_s1 = nullptr, *d_tw_s2 = nullptr;  // BR sweep outputs (atomic access only)
+  int64_t band_rows() const { return tw_a > 0 ? nt + tb : nt; }
+
   // staging of mafem_set_prescribed (grown on demand, kept)
   int64_t *d_presc_idx = nullptr;
   double *d_presc_val = nullptr;

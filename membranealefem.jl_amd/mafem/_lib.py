@@ -56,6 +56,7 @@ class Stats(ctypes.Structure):
         ("step_kernel_us", ctypes.c_double),
         ("last_relres", ctypes.c_double),
         ("sweep_groups", ctypes.c_int32), ("pivot_perturbations", ctypes.c_int32),
+        ("factor_chains", ctypes.c_int32), ("separator_rows", ctypes.c_int32),
     ]
 
 
