@@ -185,6 +185,18 @@ int mafem_get_stats(const mafem_ctx *ctx, mafem_stats *out);
  * outside the mesh (the reference would raise a BoundsError). */
 int mafem_pull_force(mafem_ctx *ctx, double *x_pull, double *f_pull);
 
+/* mafem_pull_force without allocating device memory.  No counterpart in the
+ * reference: for callers that cannot allocate (code recorded into HIP graphs,
+ * tight serving loops).  mafem_pull_force keeps the adjacent-element maps and
+ * residuals in buffers it allocates on its first call; this variant keeps
+ * them in a 64 KiB array the library holds on each device until the process
+ * ends.  Every call on a device shares that array (its contents are those of
+ * the last call): calls that overlap on one device race.  Results equal
+ * mafem_pull_force's bit for bit.  MAFEM_EUNSUPPORTED if the adjacent
+ * elements' residuals would not fit (at most 25 elements of 72 rows, 15 KB,
+ * for the reference's biquadratic meshes). */
+int mafem_pull_force_noalloc(mafem_ctx *ctx, double *x_pull, double *f_pull);
+
 /* generate_output(mesh, xms, cps) (src/Output.jl:30-118) of the device
  * state: the surface position and the unknowns sampled at every area Gauss
  * point, at the 3 Gauss points of each boundary element on the four edges and

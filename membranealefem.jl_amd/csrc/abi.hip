@@ -642,6 +642,55 @@ double linear_solve(Ctx &c, const double *d_b, double *d_x) {
 
 }  // namespace
 
+// ------------------------------------------------ allocation-free pull force
+//
+// Scratch of mafem_pull_force_noalloc: the adjacent-element list, the node
+// slot maps, the adjacent elements' residuals and the 6 outputs.  Each device
+// the library runs on holds one copy until the process ends, shared by every
+// call on that device (calls that overlap on one device race).  Host code
+// never passes `scratch` itself to a kernel or copy: its name is a host-side
+// stand-in; the device address comes from hipGetSymbolAddress, taken once
+// per device with that device current.
+constexpr size_t kScratchBytes = 64 * 1024;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxAdj = 25;  // (2 poly + 1)^2 adjacent elements for poly = 2
+
+struct PullMaps {
+  int32_t n;
+  int32_t el[kMaxAdj];
+  int8_t slot[9 * kMaxAdj];
+};
+
+__global__ void k_stage_pull_maps(PullMaps m, int32_t *__restrict__ el, int8_t *__restrict__ slot) {
+  for (int t = threadIdx.x; t < 9 * m.n; t += blockDim.x) {
+    slot[t] = m.slot[t];
+    if (t < m.n) el[t] = m.el[t];
+  }
+}
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void *g_scratch_addr[kMaxDevices] = {};
+
+// device address of `scratch` on device dev (current), looked up once per device
+unsigned char *scratch_on(int dev) {
+  if (dev < 0 || dev >= kMaxDevices)
+    throw ArgError{"device id beyond the scratch address table", MAFEM_EUNSUPPORTED};
+  std::lock_guard<std::mutex> lk(g_scratch_mutex);
+  if (!g_scratch_addr[dev]) {
+    void *p = nullptr;
+    MAFEM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw ArgError{"no device address for the scratch array", MAFEM_EHIP};
+    g_scratch_addr[dev] = p;
+  }
+  return (unsigned char *)g_scratch_addr[dev];
+}
+
+}  // namespace
+
 #define ABI_GUARD(ctx, ...)                                    \
   do {                                                         \
     if (!(ctx)) return MAFEM_EINVAL;                           \
@@ -914,9 +963,46 @@ int mafem_pull_force(mafem_ctx *ctx, double *x_pull, double *f_pull) {
     if (c.motion == STATIC) return fail(c, "f_pull needs 3-D velocity", MAFEM_EINVAL);
     MAFEM_HIP(hipSetDevice(c.dev));
     ensure_pull(c);
-    launch_pull_force(c, c.stream);
+    launch_pull_force(c, c.stream, c.d_pf_el, c.d_pf_slot, c.d_pf_rel, c.d_pf_out);
     MAFEM_HIP(hipMemcpyAsync(c.h_red, c.d_pf_out, sizeof(double) * 6, hipMemcpyDeviceToHost,
                              c.stream));
+    MAFEM_HIP(hipStreamSynchronize(c.stream));
+    for (int k = 0; k < 3; ++k) {
+      x_pull[k] = c.h_red[k];
+      f_pull[k] = c.h_red[3 + k];
+    }
+    return MAFEM_OK;
+  });
+}
+
+int mafem_pull_force_noalloc(mafem_ctx *ctx, double *x_pull, double *f_pull) {
+  ABI_GUARD(ctx, {
+    if (!x_pull || !f_pull) return fail(c, "null output", MAFEM_EINVAL);
+    if (!c.pf_err.empty()) return fail(c, c.pf_err, MAFEM_EINVAL);
+    if (c.motion == STATIC) return fail(c, "f_pull needs 3-D velocity", MAFEM_EINVAL);
+    const int64_t n = (int64_t)c.h_pf_el.size();
+    // scratch layout: el[n] at 0, slot[9 n] at 128, rel[n ne] at 384, out[6] after it
+    const size_t need = 384 + sizeof(double) * ((size_t)n * c.ne + 6);
+    if (n > kMaxAdj || need > kScratchBytes)
+      return fail(c,
+                  "pull force scratch: " + std::to_string(need) + " bytes needed, " +
+                      std::to_string(kScratchBytes) + " available (" + std::to_string(n) +
+                      " adjacent elements of " + std::to_string(c.ne) + " residual rows)",
+                  MAFEM_EUNSUPPORTED);
+    MAFEM_HIP(hipSetDevice(c.dev));
+    unsigned char *base = scratch_on(c.dev);
+    int32_t *el = (int32_t *)base;
+    int8_t *slot = (int8_t *)(base + 128);
+    double *rel = (double *)(base + 384);
+    double *out = rel + (size_t)n * c.ne;
+    PullMaps m{};
+    m.n = (int32_t)n;
+    for (int64_t k = 0; k < n; ++k) m.el[k] = c.h_pf_el[k];
+    for (int64_t k = 0; k < 9 * n; ++k) m.slot[k] = c.h_pf_slot[k];
+    hipLaunchKernelGGL(k_stage_pull_maps, dim3(1), dim3(256), 0, c.stream, m, el, slot);
+    MAFEM_HIP(hipGetLastError());
+    launch_pull_force(c, c.stream, el, slot, rel, out);
+    MAFEM_HIP(hipMemcpyAsync(c.h_red, out, sizeof(double) * 6, hipMemcpyDeviceToHost, c.stream));
     MAFEM_HIP(hipStreamSynchronize(c.stream));
     for (int k = 0; k < 3; ++k) {
       x_pull[k] = c.h_red[k];

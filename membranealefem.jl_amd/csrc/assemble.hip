@@ -797,13 +797,13 @@ int launch_assemble(Ctx &c, double time, double dt) {
   return 0;
 }
 
-void launch_pull_force(Ctx &c, hipStream_t s) {
+void launch_pull_force(Ctx &c, hipStream_t s, const int32_t *el, const int8_t *slot, double *rel,
+                       double *out) {
   const int64_t n = (int64_t)c.h_pf_el.size();
   const dim3 grid((unsigned)((n * c.ne + 255) / 256));
 #define PLIST(MO)                                                                            \
   hipLaunchKernelGGL(k_area_list<MO>, grid, dim3(256), 0, s, n, c.numnp, c.d_ix, c.d_uel,    \
-                     c.d_active, c.d_xms, c.d_cps, c.d_ufns, c.d_dbt, c.phys, c.d_pf_el,     \
-                     c.d_pf_rel)
+                     c.d_active, c.d_xms, c.d_cps, c.d_ufns, c.d_dbt, c.phys, el, rel)
   switch (c.motion) {
     case LAG: PLIST(LAG); break;
     case EUL: PLIST(EUL); break;
@@ -813,8 +813,9 @@ void launch_pull_force(Ctx &c, hipStream_t s) {
   }
 #undef PLIST
   MAFEM_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_pull_reduce, dim3(1), dim3(64), 0, s, (int)n, c.ne, c.ndf, c.d_pf_slot,
-                     c.d_pf_rel, c.d_ix + 9 * (int64_t)c.pf_pull_el, c.d_xms, c.numnp, c.d_pf_out);
+  hipLaunchKernelGGL(k_pull_reduce, dim3(1), dim3(64), 0, s, (int)n, c.ne, c.ndf, slot,
+                     (const double *)rel, c.d_ix + 9 * (int64_t)c.pf_pull_el, c.d_xms, c.numnp,
+                     out);
   MAFEM_HIP(hipGetLastError());
 }
 

@@ -718,7 +718,7 @@ int mafem_batch_pull_force(mafem_batch *bp, double *out) {
     for (int p = 0; p < P; ++p) {
       Ctx &c = b.members[p]->c;
       ensure_pull(c);
-      launch_pull_force(c, b.B.stream);
+      launch_pull_force(c, b.B.stream, c.d_pf_el, c.d_pf_slot, c.d_pf_rel, c.d_pf_out);
       MAFEM_HIP(hipMemcpyAsync(c.h_red, c.d_pf_out, sizeof(double) * 6, hipMemcpyDeviceToHost,
                                b.B.stream));
     }

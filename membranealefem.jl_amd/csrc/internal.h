@@ -313,8 +313,10 @@ std::mutex &capture_mutex();
 // kernels / launchers (assemble.hip)
 int launch_assemble(Ctx &c, double time, double dt);
 // pulled-element mean position and pull force of the current state into
-// c.d_pf_out, on stream s (device tables from ensure_pull)
-void launch_pull_force(Ctx &c, hipStream_t s);
+// out[6], on stream s: el / slot are device copies of c.h_pf_el / c.h_pf_slot,
+// rel receives the adjacent elements' residuals (h_pf_el.size() x ne)
+void launch_pull_force(Ctx &c, hipStream_t s, const int32_t *el, const int8_t *slot, double *rel,
+                       double *out);
 void ensure_pull(Ctx &c);  // abi.hip: device copies of the pull-force maps
 // generate_output sampling of the device state into c.d_out: xout
 // (Z1 x Z2 x 3) then uout (Z1 x Z2 x ndf), column-major, Zα = 3 numαel + 2

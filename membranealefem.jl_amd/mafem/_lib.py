@@ -124,6 +124,7 @@ SIGNATURES = {
                                              ctypes.c_int32, i32p, f64p, i32p]),
     "mafem_batch_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(BatchStats)]),
     "mafem_pull_force": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p]),
+    "mafem_pull_force_noalloc": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p]),
     "mafem_generate_output": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p]),
     "mafem_batch_pull_force": (ctypes.c_int, [ctypes.c_void_p, f64p]),
 }
