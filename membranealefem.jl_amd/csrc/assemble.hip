@@ -49,6 +49,9 @@ struct Lay {
   static constexpr int PMI = 7;
   // first column of the motion DOFs (get_m_motion_order, Mesh.jl:529-542)
   static constexpr int MOT0 = (MO == LAG) ? 0 : (MO == STATIC ? -1 : 3);
+  // lanes per tangent column of the area kernel: each takes the rows of
+  // ceil(9 / PARTS) nodes, so its accumulators fit the register file
+  static constexpr int PARTS = AREA_PARTS[MO];
   static constexpr int MAXE = 256 / NE + 2;  // elements a 256-lane block can touch
 };
 
@@ -96,8 +99,10 @@ __device__ __forceinline__ void geometry(const double *X /*[3][9]*/, const doubl
 
 // LIST = false: elements 0..numel-1, outputs K_el and r_el per element.
 // LIST = true: the numel elements elist[0..numel-1], output r_el only, at the
-// element's position in the list.
-template <int MO, bool LIST = false>
+// element's position in the list.  P lanes share each tangent column (lane
+// part p takes the rows of nodes p NPP .. p NPP + NPP - 1, NPP = ceil(9 / P));
+// the part holding the column's own row writes its r_el entry.
+template <int MO, bool LIST = false, int P = LIST ? 1 : Lay<MO>::PARTS>
 __device__ __forceinline__ void area_body(
     int64_t bid, int64_t numel, int64_t numnp, const int32_t *__restrict__ ix,
     const int32_t *__restrict__ uel, const uint8_t *__restrict__ active,
@@ -107,14 +112,16 @@ __device__ __forceinline__ void area_body(
     const int32_t *__restrict__ elist = nullptr) {
   using L = Lay<MO>;
   auto EL = [&](int64_t i) -> int64_t { return LIST ? (int64_t)elist[i] : i; };
-  constexpr int NDF = L::NDF, NE = L::NE, MAXE = L::MAXE;
+  constexpr int NDF = L::NDF, NE = L::NE;
+  constexpr int NPP = (9 + P - 1) / P, NR = NPP * NDF, LPE = NE * P;
+  constexpr int MAXE = 256 / LPE + 2;  // elements a 256-lane block can touch
   __shared__ double sX[MAXE][3 * 9];
   __shared__ double sC[MAXE][NDF * 9];
   __shared__ double sB[MAXE][9 * RECP];
 
   const int64_t f0 = bid * 256;
-  const int64_t e_lo = f0 / NE;
-  int64_t e_hi = (f0 + 255) / NE;
+  const int64_t e_lo = f0 / LPE;
+  int64_t e_hi = (f0 + 255) / LPE;
   if (e_hi > numel - 1) e_hi = numel - 1;
   const int nle = (int)(e_hi - e_lo + 1);
   // stage element state: positions and control points of the 9 nodes
@@ -135,12 +142,15 @@ __device__ __forceinline__ void area_body(
   __syncthreads();
 
   const int64_t f = f0 + threadIdx.x;
-  const bool valid = f < numel * NE;
-  const int64_t e = valid ? f / NE : e_hi;
+  const bool valid = f < numel * LPE;
+  const int64_t e = valid ? f / LPE : e_hi;
   const int64_t eg = EL(e);
+  const int part = (int)(f % LPE) / NE;
   const int col = (int)(f % NE);
+  const int a_lo = part * NPP;  // first node of this lane's rows
   const int le = (int)(e - e_lo);
   const int a0 = col / NDF, d0 = col % NDF;
+  const bool own_row = a0 >= a_lo && a0 < a_lo + NPP;
   const int64_t node0 = ix[eg * 9 + a0];
   const bool act = active[d0 + NDF * node0] != 0;
   const double sC0 = act ? 1.0 : 0.0;
@@ -150,9 +160,9 @@ __device__ __forceinline__ void area_body(
   const double *X = sX[le];
   const double *C = sC[le];
 
-  double acc[NE];
+  double acc[NR];
 #pragma unroll
-  for (int k = 0; k < NE; ++k) acc[k] = 0.0;
+  for (int k = 0; k < NR; ++k) acc[k] = 0.0;
   double racc = 0.0;
   const double azv = ph.adb / ph.zv;
 
@@ -316,15 +326,18 @@ __device__ __forceinline__ void area_body(
       pmJ = pm * wJ;
     }
 
-#define ADD(row, val)                          \
-  do {                                         \
-    const D v_ = (val);                        \
-    acc[(row)] += v_.t;                        \
-    racc += ((row) == col) ? v_.v : 0.0;       \
+// local row j NDF + r of this lane is row a NDF + r of the element
+#define ADD(r, val)                                  \
+  do {                                               \
+    const D v_ = (val);                              \
+    acc[j * NDF + (r)] += v_.t;                      \
+    racc += (a * NDF + (r) == col) ? v_.v : 0.0;     \
   } while (0)
 
 #pragma unroll
-    for (int a = 0; a < 9; ++a) {
+    for (int j = 0; j < NPP; ++j) {
+      const int a = a_lo + j;
+      if (9 % P != 0 && a >= 9) break;
       const double n1 = bN1[a], n2 = bN2[a], na = bN[a];
       D s1 = n1 * sg[0] + (0.5 * n2) * sg[2];
       D s2 = n2 * sg[1] + (0.5 * n1) * sg[2];
@@ -336,12 +349,12 @@ __device__ __forceinline__ void area_body(
       for (int i = 0; i < L::NV; ++i) {
         D rv = A1[i] * s1 + A2[i] * s2 + Nn[i] * m;
         if (ph.pn != 0.0) rv = rv - (ph.pn * na) * Nn[i];
-        ADD(a * NDF + i, rv);
+        ADD(i, rv);
       }
-      ADD(a * NDF + L::LAMI, na * ql);
+      ADD(L::LAMI, na * ql);
       if constexpr (MO == EUL) {
 #pragma unroll
-        for (int i = 0; i < 3; ++i) ADD(a * NDF + 3 + i, na * gm[i]);
+        for (int i = 0; i < 3; ++i) ADD(3 + i, na * gm[i]);
       }
       if constexpr (L::HAS_PM) {
         D t1 = n1 * sgm[0] + (0.5 * n2) * sgm[2];
@@ -351,9 +364,9 @@ __device__ __forceinline__ void area_body(
           D rm = A1[i] * t1 + A2[i] * t2;
           if constexpr (MO == ALEVB) rm = rm + Nn[i] * m;
           rm = rm - na * (pmJ * g.n[i]);
-          ADD(a * NDF + 3 + i, rm);
+          ADD(3 + i, rm);
         }
-        ADD(a * NDF + L::PMI, na * qp);
+        ADD(L::PMI, na * qp);
       }
     }
 #undef ADD
@@ -364,15 +377,17 @@ __device__ __forceinline__ void area_body(
     const double *T = dbt + (int64_t)uel[eg] * 81;
     if (d0 == L::LAMI && act) {
 #pragma unroll
-      for (int a = 0; a < 9; ++a) acc[a * NDF + L::LAMI] += T[a * 9 + a0] * azv;
+      for (int j = 0; j < NPP; ++j)
+        if (9 % P == 0 || a_lo + j < 9) acc[j * NDF + L::LAMI] += T[(a_lo + j) * 9 + a0] * azv;
     }
     if constexpr (L::HAS_PM) {
       if (d0 == L::PMI && act) {
 #pragma unroll
-        for (int a = 0; a < 9; ++a) acc[a * NDF + L::PMI] += T[a * 9 + a0] * azv;
+        for (int j = 0; j < NPP; ++j)
+          if (9 % P == 0 || a_lo + j < 9) acc[j * NDF + L::PMI] += T[(a_lo + j) * 9 + a0] * azv;
       }
     }
-    if (d0 == L::LAMI || (L::HAS_PM && d0 == L::PMI)) {
+    if (own_row && (d0 == L::LAMI || (L::HAS_PM && d0 == L::PMI))) {
       double s = 0;
       for (int b = 0; b < 9; ++b) s = fma(T[a0 * 9 + b], C[d0 * 9 + b], s);
       racc += s * azv;
@@ -383,9 +398,13 @@ __device__ __forceinline__ void area_body(
     if constexpr (!LIST) {
       double *K = kel + e * (int64_t)NE * NE;
 #pragma unroll
-      for (int row = 0; row < NE; ++row) K[row * NE + col] = acc[row];
+      for (int j = 0; j < NPP; ++j)
+        if (9 % P == 0 || a_lo + j < 9) {
+#pragma unroll
+          for (int r = 0; r < NDF; ++r) K[((a_lo + j) * NDF + r) * NE + col] = acc[j * NDF + r];
+        }
     }
-    rel[e * NE + col] = racc;
+    if (own_row) rel[e * NE + col] = racc;
   }
 }
 
@@ -742,7 +761,7 @@ __global__ __launch_bounds__(256) void kb_predict(Seg sg, const PatchDev *__rest
 }
 
 static hipError_t launch_area(Ctx &c, double dt) {
-  const int64_t lanes = c.numel * c.ne;
+  const int64_t lanes = c.numel * c.ne * area_parts(c.motion);
   const dim3 grid((unsigned)((lanes + 255) / 256));
 #define AREA(MO)                                                                        \
   hipLaunchKernelGGL(k_area<MO>, grid, dim3(256), 0, c.stream, c.numel, c.numnp, c.d_ix, \

@@ -238,7 +238,7 @@ void build_plan(mafem_batch &b) {
     for (int p = 0; p < P; ++p) {
       const PatchDev &d = b.h_pd[p];
       if (d.motion != mo) continue;
-      ba[p] = cdiv(d.numel * d.ne, 256);
+      ba[p] = cdiv(d.numel * d.ne * area_parts(d.motion), 256);
       be[p] = cdiv(d.n_edges * d.ne, 256);
     }
     B.area.push_back(make_seg(b, ba, 256));

@@ -4,8 +4,10 @@ Every patch is the reference's F_PULL scenario (src/input/Bc.jl:188-269,
 src/input/Input.jl:77-108) on a flat square of side 1 with its own size,
 formulation and parameters; the pull is the prescribed velocity of the pulled
 element's control points, ramped linearly in time to the drawn magnitude.
-Meshes depend only on (motion, n), so they are built once per pair and shared.
+Mesh tables depend only on (motion, n), so they are built once per pair and
+shared; each patch's copy carries its own edge traction kb / 4.
 """
+import copy
 from dataclasses import dataclass
 
 import numpy as np
@@ -64,12 +66,19 @@ def serving_specs(count, seed=0xC5, sizes=(32, 48, 64, 96, 128), motions=(LAG, E
 _MESHES = {}
 
 
-def mesh_for(motion, n):
+def mesh_for(motion, n, kb=1.0):
+    """The F_PULL mesh of (motion, n) for bending modulus kb.  The mesh tables
+    depend only on (motion, n) and are built once per pair; the Neumann edge
+    traction is the patch's own kb / 4 (src/input/Bc.jl:259-264), as
+    generate_mesh would give it for the patch's Params."""
     key = (int(motion), int(n))
     if key not in _MESHES:
         p = Params(motion=motion, num1el=n, num2el=n, length=1.0, enr=1e-10, output=False)
         _MESHES[key] = make_mesh(p, pull_speed=0.0)
-    return _MESHES[key]
+    m = copy.copy(_MESHES[key])
+    m.inh_neu_bcs = [(b, t, kb / 4) for (b, t, _v) in m.inh_neu_bcs]
+    m._ctx = None
+    return m
 
 
 class Patch:
@@ -79,7 +88,7 @@ class Patch:
         self.spec = spec
         self.p = Params(motion=spec.motion, num1el=spec.n, num2el=spec.n, length=1.0,
                         kb=spec.kb, zv=spec.zv, kg=-0.5, enr=1e-10, output=False)
-        self.mesh = mesh_for(spec.motion, spec.n)
+        self.mesh = mesh_for(spec.motion, spec.n, spec.kb)
         xms, cps = prepare_input(self.p, self.mesh, pull_speed=0.0)
         if sigma > 0.0:  # optional seeded out-of-plane perturbation
             rng = np.random.default_rng(spec.seed)
@@ -113,7 +122,8 @@ def device_bytes(spec: PatchSpec, precision=32, restart=20):
     tb = min(-(-((2 * n1 + 3) * ndf - 1) // 64), max(1, nt - 1))
     ne = 9 * ndf
     es = 4 if precision == 32 else 8
-    band = nt * (2 * tb + 1) * 64 * 64 * es + 2 * nt * 64 * 64 * es
+    rows = nt + tb if nt >= 8 * tb else nt  # the two-ended layout keeps tb more tile rows
+    band = rows * (2 * tb + 1) * 64 * 64 * es + 2 * rows * 64 * 64 * es
     nnzb = numnp * 25
     kel = (numel + 4 * n) * ne * ne * 8
     bsr = nnzb * ndf * ndf * 8

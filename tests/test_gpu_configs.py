@@ -48,7 +48,9 @@ def test_c2_ramp_matches_oracle_trajectory():
     xms, cps = mafem.prepare_input(p, mesh, pull_speed=0.0)
     rng = np.random.default_rng(0xC2)
     xms[:, 2] += rng.normal(0.0, 1e-6, mesh.numnp)
-    assert float(xms.sum()) == pytest.approx(float(g["x0_sum"]), rel=1e-15, abs=1e-15)
+    # the same initial state as the oracle's (positions agree to rounding:
+    # the two collocation fits sum in different orders)
+    assert abs(float(xms.sum()) - float(g["x0_sum"])) <= 1e-12
     assert float(cps.sum()) == pytest.approx(float(g["c0_sum"]), rel=1e-15)
     dev = mafem.DeviceMesh(mesh, p)
     dev.set_state(xms, cps)
