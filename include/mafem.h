@@ -145,6 +145,16 @@ int mafem_get_state(mafem_ctx *ctx, double *xms, double *cps);
 int mafem_set_prescribed(mafem_ctx *ctx, int64_t n, const int64_t *node,
                          const int32_t *dof, const double *val);
 
+/* Replaces the equation numbering (the reference's `mesh.ID`, ndf x numnp,
+ * column-major, 1-based, 0 = prescribed; Mesh.jl:72, built by
+ * generate_scenario, Mesh.jl:262-302) of a context whose mesh is otherwise
+ * unchanged, e.g. when the set of prescribed unknowns moves with a tether
+ * that translates across the patch.  The reference rebuilds its Mesh for a
+ * new Dirichlet set (restart with new arguments, src/MembraneAleFem.jl:75-107);
+ * the state stays on the device, prescribed values are set with
+ * mafem_set_prescribed.  Not while the context belongs to a batch. */
+int mafem_set_dof_ids(mafem_ctx *ctx, const int64_t *ID, int64_t nmdf);
+
 /* update_xms!(motion, xms, cps, Δt, dofs): the predictor x += Δt u_motion
  * applied before each step (FiniteElement.jl:408-423, Analysis.jl:70). */
 int mafem_predict(mafem_ctx *ctx, double dt);

@@ -814,6 +814,47 @@ int mafem_set_prescribed(mafem_ctx *ctx, int64_t n, const int64_t *node, const i
   });
 }
 
+int mafem_set_dof_ids(mafem_ctx *ctx, const int64_t *ID, int64_t nmdf) {
+  ABI_GUARD(ctx, {
+    std::lock_guard<std::mutex> no_capture(capture_mutex());
+    if (!ID || nmdf < 1) return fail(c, "null ID or nmdf < 1", MAFEM_EINVAL);
+    const int64_t n = c.ndf * c.numnp;
+    std::vector<uint8_t> active(n);
+    std::vector<int64_t> eq(nmdf, -1);
+    for (int64_t k = 0; k < n; ++k) {
+      const int64_t id = ID[k];
+      if (id < 0 || id > nmdf || (id > 0 && eq[id - 1] >= 0))
+        return fail(c, "ID is not a numbering of 1..nmdf", MAFEM_EINVAL);
+      active[k] = id != 0;
+      if (id > 0) eq[id - 1] = k;
+    }
+    for (int64_t v : eq)
+      if (v < 0) return fail(c, "ID is not a numbering of 1..nmdf", MAFEM_EINVAL);
+    MAFEM_HIP(hipSetDevice(c.dev));
+    MAFEM_HIP(hipStreamSynchronize(c.stream));
+    MAFEM_HIP(hipMemcpy(c.d_active, active.data(), n, hipMemcpyHostToDevice));
+    if (nmdf != c.nmdf) {
+      if (c.d_eq) (void)hipFree(c.d_eq);
+      c.d_eq = nullptr;
+      c.d_eq = dalloc<int64_t>(c, nmdf);
+    }
+    MAFEM_HIP(hipMemcpy(c.d_eq, eq.data(), sizeof(int64_t) * nmdf, hipMemcpyHostToDevice));
+    c.nmdf = nmdf;
+    c.h_ID.assign(ID, ID + n);
+    c.h_eq = eq;
+    // the CSC export pattern follows the numbering: rebuilt on next use
+    if (c.d_csc_src) (void)hipFree(c.d_csc_src);
+    if (c.d_csc_val) (void)hipFree(c.d_csc_val);
+    c.d_csc_src = nullptr;
+    c.d_csc_val = nullptr;
+    c.nnz = -1;
+    c.h_colptr.clear();
+    c.h_rowval.clear();
+    c.have_factor = false;  // a kept preconditioner belongs to the old Dirichlet set
+    return MAFEM_OK;
+  });
+}
+
 int mafem_predict(mafem_ctx *ctx, double dt) {
   ABI_GUARD(ctx, {
     MAFEM_HIP(hipSetDevice(c.dev));

@@ -104,6 +104,7 @@ SIGNATURES = {
     "mafem_get_state": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p]),
     "mafem_set_prescribed": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, i64p, i32p, f64p]),
     "mafem_predict": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
+    "mafem_set_dof_ids": (ctypes.c_int, [ctypes.c_void_p, i64p, ctypes.c_int64]),
     "mafem_nnz": (ctypes.c_int, [ctypes.c_void_p, i64p]),
     "mafem_assemble": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, f64p,
                                       i64p, i64p, f64p]),

@@ -115,6 +115,15 @@ class DeviceMesh:
     def predict(self, dt):
         self._check(self.lib.mafem_predict(self.h, float(dt)))
 
+    def set_dof_ids(self, ID):
+        """Replace the equation numbering (mesh.ID, ndf x numnp, 0 = prescribed)
+        of this context; its mesh and state are kept."""
+        a = np.ascontiguousarray(np.asarray(ID, dtype=np.int64).ravel(order="F"))
+        nmdf = int(a.max())
+        self._check(self.lib.mafem_set_dof_ids(self.h, L.ptr(a, L.i64p), nmdf))
+        self.nmdf = nmdf
+        self._nnz = None
+
     # -- hot path
     def nnz(self):
         if self._nnz is None:
@@ -330,6 +339,39 @@ def get_adj_maps(num1el, numel, IX, poly=2):
         ra = [3 * b + c for b in range(9) if adj[b] in pull_nodes for c in range(3)]
         maps.append((rp, ra))
     return ids, maps
+
+
+def lateral_pull_element(mesh, k, steps, fraction=0.4):
+    """Tethered element at time step k (0-based) of a tether translated across
+    `fraction` of the patch width over `steps` steps: the reference's element
+    ceil(numel / 2) (which lies on the right edge for an even num1el) moved
+    toward decreasing ζ1 by round(fraction num1el k / steps) elements.  An
+    extension of this build: the reference has no moving tether."""
+    shift = int(round(fraction * mesh.num1el * k / steps))
+    return get_pull_el_id(mesh.numel) - shift
+
+
+def move_tether(dev, mesh, p, old_el, new_el, speed):
+    """Moves the F_PULL tether of a device context from element old_el to
+    new_el (1-based): the context takes the numbering of the new Dirichlet
+    set (tether_ids), the new tethered nodes get v_z (and v^m_z off LAG) =
+    speed and their other velocity unknowns 0, and the released nodes'
+    unknowns that stay prescribed return to 0; unknowns that become free keep
+    their values.  Returns the new (ID, inh_dir_bcs)."""
+    from .mesh import tether_ids
+    ID, dirs = tether_ids(mesh, p, new_el, speed)
+    dev.set_dof_ids(ID)
+    inh = {(int(node), mesh.dofs[u]) for (u, node, _v) in dirs}
+    nodes = set(int(v) for v in mesh.IX[:, old_el - 1]) | set(int(v) for v in mesh.IX[:, new_el - 1])
+    pn, pd, pv = [], [], []
+    for node in sorted(nodes):
+        for d in range(1, mesh.ndf + 1):
+            if ID[d - 1, node - 1] == 0:
+                pn.append(node)
+                pd.append(d)
+                pv.append(float(speed) if (node, d) in inh else 0.0)
+    dev.set_prescribed(pn, pd, pv)
+    return ID, dirs
 
 
 def calc_pull_force(mesh, xms, cps, adj_el_ids, adj_node_map, p, **args):

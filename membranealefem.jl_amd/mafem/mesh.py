@@ -313,6 +313,22 @@ def make_mesh(p, **args):
                 dir_bcs, neu_bcs, crnr_ufns)
 
 
+def tether_ids(mesh, p, pull_el, pull_speed):
+    """(ID, inh_dir_bcs) of the F_PULL scenario on `mesh` with the tether on
+    element `pull_el` (1-based) instead of ceil(numel / 2): the numbering and
+    Dirichlet list generate_scenario would build (Mesh.jl:262-302,
+    Bc.jl:188-269) for that element.  An extension of this build (the
+    reference has no moving tether); pull_el = ceil(numel / 2) gives mesh.ID."""
+    _, ndf, fixed, dirs, _ = _scenario_bcs(p, mesh.numnp, mesh.IX, mesh.bdry_nodes,
+                                           mesh.bdry_inner_nodes, mesh.crnr_nodes,
+                                           pull_speed=pull_speed, pull_el=pull_el)
+    free = ~fixed
+    ID = np.zeros((ndf, mesh.numnp), dtype=np.int64, order="F")
+    flat = free.ravel(order="F")
+    ID.ravel(order="F")[:] = np.where(flat, np.cumsum(flat), 0)
+    return ID, dirs
+
+
 def _scenario_bcs(p, nn, ix, bn, bin_, crnr, **args):
     """get_scenario_bc_info (Bc.jl:15-393): DOFs, fixed mask, Dirichlet and
     Neumann lists."""
@@ -339,7 +355,11 @@ def _scenario_bcs(p, nn, ix, bn, bin_, crnr, **args):
         fix(U.vz, alli)
         if p.motion == ALEVB:
             fix(U.vmz, alli)
-        for node in ix[:, get_pull_el_id(ix.shape[1]) - 1]:
+        # `pull_el` (1-based; an extension of this build, no reference
+        # counterpart) moves the tether; the reference pulls element
+        # ceil(numel / 2)
+        pe = int(args.get("pull_el", get_pull_el_id(ix.shape[1])))
+        for node in ix[:, pe - 1]:
             node = int(node)
             fix(U.vx, [node]); fix(U.vy, [node]); fix(U.vz, [node])
             dirs.append((U.vz, node, speed))

@@ -213,6 +213,14 @@ def _generate_scenario(m, p, **args):
     m.inh_dir_bcs, m.inh_neu_bcs = dir_bcs, neu_bcs
 
 
+def retarget_pull(m, p, pull_el, pull_speed):
+    """The F_PULL numbering and Dirichlet lists of mesh m rebuilt, in place,
+    with the tether on element pull_el (1-based): generate_scenario
+    (Mesh.jl:262-302) with the pulled element of Bc.jl:227-243 replaced
+    (extension; the reference always pulls ceil(numel / 2))."""
+    _generate_scenario(m, p, pull_speed=pull_speed, pull_el=pull_el)
+
+
 def _scenario_bc_info(m, p, **args):
     """get_scenario_bc_info and the per-scenario functions (Bc.jl:15-393)."""
     bn = m.bdry_nodes
@@ -278,7 +286,8 @@ def _scenario_bc_info(m, p, **args):
                 mark(VZ, i)
                 if p.motion == ALEVB:
                     mark(VMZ, i)
-        pe = get_pull_el_id(m.numel)
+        # `pull_el` moves the tether (extension, no reference counterpart)
+        pe = int(args.get("pull_el", get_pull_el_id(m.numel)))
         for node in m.IX[:, pe - 1]:
             node = int(node)
             mark(VX, node)
