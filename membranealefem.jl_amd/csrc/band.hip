@@ -174,13 +174,16 @@ __global__ __launch_bounds__(256) void k_rev_copy(int64_t k0, int64_t n, int64_t
 
 // Maximum over the wavefront of a 32-bit key, delivered as a scalar: DPP
 // row-local steps, then row broadcasts that leave the total in lane 63.
+// max over the wavefront.  The moved-in operand is 0 (the identity of the
+// unsigned max) where a row is masked off, which lets the compiler fold each
+// move into v_max_u32_dpp: one instruction per stage.
 __device__ __forceinline__ unsigned wave_max_u32(unsigned x) {
-  x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0xB1, 0xF, 0xF, false));
-  x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x4E, 0xF, 0xF, false));
-  x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x141, 0xF, 0xF, false));
-  x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x140, 0xF, 0xF, false));
-  x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x142, 0xA, 0xF, false));
-  x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x143, 0xC, 0xF, false));
+  x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, true));
+  x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xF, 0xF, true));
+  x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x141, 0xF, 0xF, true));
+  x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x140, 0xF, 0xF, true));
+  x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, true));
+  x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, true));
   return (unsigned)__builtin_amdgcn_readlane((int)x, 63);
 }
 
@@ -330,13 +333,6 @@ __device__ __forceinline__ void diag_body(int64_t K, int tb, S *__restrict__ ban
   for (int i = 0; i < T; ++i) st[i][lane] = A[i * T + lane];
   __syncthreads();
   diag_core<S, PERT>(K, A, piv, st, u12, l11, amax);
-}
-
-template <class S, bool PERT>
-__global__ __launch_bounds__(64) void k_diag(int64_t K, int tb, S *__restrict__ band,
-                                             int32_t *__restrict__ piv,
-                                             unsigned long long *__restrict__ amax) {
-  diag_body<S, PERT>(K, tb, band, piv, amax);
 }
 
 // amax[0] = max |a| over n values (non-negative doubles order like their bits)
@@ -579,6 +575,156 @@ __device__ __forceinline__ void update_body(int64_t bid, int64_t K, int64_t nt, 
     for (int j = 0; j < 4; ++j) A[mm_row<S>(j) * T + mm_col<S>(bj)] = old[bj][j] - c[bj][j];
 }
 
+// accumulator row j of lane l in a 16 x 16 MFMA block (the FP64 and FP32
+// instructions lay their results out differently)
+template <class S>
+__device__ __forceinline__ int acc_row(int j) {
+  const int g = (int)((threadIdx.x & 63) >> 4);
+  return sizeof(S) == 8 ? g + 4 * j : 4 * g + j;
+}
+
+// LU with partial pivoting of the 64x64 diagonal tile by a 256-thread block.
+// The tile is in st (st[i][0..63] = row i, original order).  Panels of 16
+// columns: the first wavefront holds the panel's entries of every row in
+// registers (lane = row) and runs its 16 pivot steps exactly as diag_core
+// does (rows are not moved; the row chosen at step j becomes row j of the
+// factors); the panel's pivot rows then solve U12 = L11^-1 A12 (one trailing
+// column per lane), and the four wavefronts subtract L21 U12 from the rows
+// still unchosen on the matrix cores (wavefront w: rows 16w..16w+15; chosen
+// rows enter with a zero multiplier and are not written).  Only the pivot
+// steps stay on one wavefront.  The factored tile goes to A in pivot order,
+// the pivots to piv[K]; prow / cand: 64 ints of LDS each (the row chosen at
+// step j; whether row i is still a candidate).  amax / PERT: static
+// pivoting as in diag_core.
+template <class S, bool PERT = false>
+__device__ __forceinline__ void diag_core_mm(int64_t K, S *__restrict__ A, int32_t *__restrict__ piv,
+                                             S (*st)[T + 1], int *prow, int *cand,
+                                             unsigned long long *amax = nullptr) {
+  using V = typename Mfma<S>::V;
+  const int t = (int)threadIdx.x, lane = t & 63, w = t >> 6;
+  const int g = lane >> 4, cc = lane & 15;
+  S thr = S(0);
+  if (PERT) {
+    const double rel = sizeof(S) == 8 ? 2.220446049250313e-16 : 1.1920928955078125e-7;
+    thr = (S)(rel * __longlong_as_double((long long)__hip_atomic_load(
+                           amax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
+  }
+  int npert = 0;
+  int pos = -1;
+#pragma unroll 1
+  for (int pb = 0; pb < 4; ++pb) {
+    const int c0 = pb * 16;
+    if (w == 0) {
+      S r[16];
+#pragma unroll
+      for (int jj = 0; jj < 16; ++jj) r[jj] = st[lane][c0 + jj];
+#pragma unroll
+      for (int jj = 0; jj < 16; ++jj) {
+        const unsigned key = wave_max_u32(pivot_key(r[jj], pos < 0, lane));
+        const int p = 63 - (int)(key & 63u);
+        S pj = lane_bcast(r[jj], p);
+        bool tiny = false;
+        if (PERT) {  // wave-uniform
+          tiny = fabs(pj) < thr;
+          pj = tiny ? (pj < S(0) ? -thr : thr) : pj;
+          npert += tiny ? 1 : 0;
+        }
+        if (lane == p) pos = c0 + jj;
+        const S l = (pos < 0) ? r[jj] / pj : S(0);
+        if (pos < 0) r[jj] = l;
+        if (PERT && tiny && lane == p) r[jj] = pj;
+#pragma unroll
+        for (int k = jj + 1; k < 16; ++k) r[k] = fma(-l, lane_bcast(r[k], p), r[k]);
+      }
+#pragma unroll
+      for (int jj = 0; jj < 16; ++jj) st[lane][c0 + jj] = r[jj];
+      if (pos >= c0) prow[pos] = lane;
+      cand[lane] = pos < 0 ? 1 : 0;
+    }
+    __syncthreads();
+    if (pb == 3) break;
+    if (w == 0 && lane < 48 - c0) {  // U12 = L11^-1 A12, lane = trailing column
+      const int c = c0 + 16 + lane;
+      S u[16];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int ri = prow[c0 + i];
+        S acc = st[ri][c];
+#pragma unroll
+        for (int ii = 0; ii < i; ++ii) acc = fma(-st[ri][c0 + ii], u[ii], acc);
+        u[i] = acc;
+      }
+#pragma unroll
+      for (int i = 0; i < 16; ++i) st[prow[c0 + i]][c] = u[i];
+    }
+    __syncthreads();
+    {  // A22 -= L21 U12 on the matrix cores, wavefront w: rows 16w .. 16w + 15
+      const int row = 16 * w + cc;
+      const bool cd = cand[row] != 0;
+      S a[4];
+      int br[4];
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+        const int k = 4 * g + kk;
+        a[kk] = cd ? -st[row][c0 + k] : S(0);
+        br[kk] = prow[c0 + k];
+      }
+      bool wr[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) wr[j] = cand[16 * w + acc_row<S>(j)] != 0;
+#pragma unroll 1
+      for (int cb = 0; cb < 3 - pb; ++cb) {
+        const int C0 = c0 + 16 + 16 * cb;
+        V acc;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[j] = st[16 * w + acc_row<S>(j)][C0 + cc];
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) acc = Mfma<S>::run(a[kk], st[br[kk]][C0 + cc], acc);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (wr[j]) st[16 * w + acc_row<S>(j)][C0 + cc] = acc[j];
+      }
+    }
+    __syncthreads();
+  }
+  if (w == 0) {
+    piv[K * T + pos] = lane;
+    if (PERT && npert > 0 && lane == 0) atomicAdd(amax + 1, (unsigned long long)npert);
+  }
+#pragma unroll
+  for (int it = 0; it < T / 4; ++it) {
+    const int i = 4 * it + w;
+    A[i * T + lane] = st[prow[i]][lane];
+  }
+}
+
+// diag_core_mm on the tile staged in the first T (T + 1) entries of an LDS
+// buffer of MM_LDS entries (its index arrays follow the tile)
+template <class S, bool PERT>
+__device__ __forceinline__ void diag_mm_lds(int64_t K, S *__restrict__ A, int32_t *__restrict__ piv,
+                                            S *lds, unsigned long long *amax) {
+  S(*st)[T + 1] = reinterpret_cast<S(*)[T + 1]>(lds);
+  int *prow = reinterpret_cast<int *>(lds + T * (T + 1));
+  diag_core_mm<S, PERT>(K, A, piv, st, prow, prow + T, amax);
+}
+
+// the first diagonal LU of a chain: tile (K, K) of the band, 256 threads
+template <class S, bool PERT>
+__global__ __launch_bounds__(256) void k_diag_mm(int64_t K, int tb, S *__restrict__ band,
+                                                 int32_t *__restrict__ piv,
+                                                 unsigned long long *__restrict__ amax) {
+  __shared__ __align__(16) S lds[MM_LDS];
+  S *A = tile_ptr(band, tb, K, K);
+  const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+#pragma unroll
+  for (int it = 0; it < T / 4; ++it) {
+    const int i = 4 * it + w;
+    lds[i * (T + 1) + lane] = A[i * T + lane];
+  }
+  __syncthreads();
+  diag_mm_lds<S, PERT>(K, A, piv, lds, amax);
+}
+
 template <class S>
 __global__ __launch_bounds__(256) void k_update(int64_t K, int64_t nt, int tb, S *__restrict__ band) {
   __shared__ __align__(16) S lds[MM_LDS];
@@ -597,10 +743,7 @@ __global__ __launch_bounds__(256) void k_update_diag(int64_t K, int64_t nt, int 
   update_body<S>((int64_t)blockIdx.x, K, nt, tb, band, lds, fused);
   if (!fused) return;
   __syncthreads();
-  S(*st)[T + 1] = reinterpret_cast<S(*)[T + 1]>(lds);
-  S(*u12)[T] = reinterpret_cast<S(*)[T]>(lds + T * (T + 1));
-  S(*l11)[16] = reinterpret_cast<S(*)[16]>(lds + T * (T + 1) + 16 * T);
-  diag_core<S, PERT>(K + 1, tile_ptr(band, tb, K + 1, K + 1), piv, st, u12, l11, amax);
+  diag_mm_lds<S, PERT>(K + 1, tile_ptr(band, tb, K + 1, K + 1), piv, lds, amax);
 }
 
 // Both chains of the two-ended factorization in one launch (blocks of chain 1
@@ -636,10 +779,7 @@ __global__ __launch_bounds__(256) void k_update_diag2(int tb, FacChain c0, FacCh
   update_body<S>(bid, ch.K, ch.nt, tb, band, lds, fused);
   if (!fused) return;
   __syncthreads();
-  S(*st)[T + 1] = reinterpret_cast<S(*)[T + 1]>(lds);
-  S(*u12)[T] = reinterpret_cast<S(*)[T]>(lds + T * (T + 1));
-  S(*l11)[16] = reinterpret_cast<S(*)[16]>(lds + T * (T + 1) + 16 * T);
-  diag_core<S, PERT>(ch.K + 1, tile_ptr(band, tb, ch.K + 1, ch.K + 1), ch.piv, st, u12, l11, amax);
+  diag_mm_lds<S, PERT>(ch.K + 1, tile_ptr(band, tb, ch.K + 1, ch.K + 1), ch.piv, lds, amax);
 }
 
 // ------------------------------------------------- post-factorization pass
@@ -1436,7 +1576,7 @@ template <class S, bool PERT>
 static void factor_chain(Ctx &c, hipStream_t s, S *band, int32_t *piv, int64_t nt, int64_t k0,
                          int64_t k1, bool diag_first, bool fuse_last, bool timed) {
   if (diag_first)
-    hipLaunchKernelGGL((k_diag<S, PERT>), dim3(1), dim3(64), 0, s, k0, c.tb, band, piv, c.d_amax);
+    hipLaunchKernelGGL((k_diag_mm<S, PERT>), dim3(1), dim3(256), 0, s, k0, c.tb, band, piv, c.d_amax);
   const int64_t timed_K = timed ? (k0 + k1) / 2 : -1;
   for (int64_t K = k0; K < k1; ++K) {
     hipLaunchKernelGGL(k_panel<S>, dim3(3 * c.tb), dim3(64), 0, s, K, nt, c.tb, band, piv);
@@ -1478,9 +1618,9 @@ static void factor_run(Ctx &c, S *band, S *li, S *ui) {
   const int64_t tsz = (int64_t)T * T;
   S *br = band + rt * (2 * c.tb + 1) * tsz;
   int32_t *pr = c.d_piv + rt * T;
-  hipLaunchKernelGGL((k_diag<S, PERT>), dim3(1), dim3(64), 0, c.stream, (int64_t)0, c.tb, band,
+  hipLaunchKernelGGL((k_diag_mm<S, PERT>), dim3(1), dim3(256), 0, c.stream, (int64_t)0, c.tb, band,
                      c.d_piv, c.d_amax);
-  hipLaunchKernelGGL((k_diag<S, PERT>), dim3(1), dim3(64), 0, c.stream, (int64_t)0, c.tb, br, pr,
+  hipLaunchKernelGGL((k_diag_mm<S, PERT>), dim3(1), dim3(256), 0, c.stream, (int64_t)0, c.tb, br, pr,
                      c.d_amax);
   const int64_t steps = std::max(a, nb), timed_k = std::min(a, nb) / 2;
   for (int64_t k = 0; k < steps; ++k) {
