@@ -58,6 +58,10 @@ STEP_COUNTERS_KIB = {
     "k_update_diag": (8914.95051125919, 9064.25),
     "k_update_diag2": None,
 }
+# the same counters per dispatch of the area kernel of the bench patch, by motion
+AREA_COUNTERS_KIB = {
+    "ALEVB": None,
+}
 
 
 def band_update_flops(dev):
@@ -356,13 +360,14 @@ def bench_precision(args):
     across 40 % of the patch width over 500 steps (an extension: the reference
     pulls a fixed element; the tethered element steps toward decreasing
     zeta_1 from ceil(numel / 2), mafem.analysis.lateral_pull_element), pull
-    speed ramped as in the plain workload.  It is advanced with the
-    mixed-precision solver (FP64 FGMRES right-preconditioned by an FP32 band
-    LU of the current tangent, refactored for every Newton iteration; FP64
-    assembly, residual, update and epsilon test) and, from the same initial
-    state, with the all-FP64 direct path; reports the per-step relative state
-    error between the two and both throughputs.  Step 0 (which includes the
-    sweep autotuner) is run but not timed."""
+    speed ramped as in the plain workload.  It is advanced with the all-FP64
+    direct path for --steps steps (the timed `value`) and, from the same
+    initial state, with the mixed-precision solver (FP64 FGMRES
+    right-preconditioned by an FP32 band LU of the current tangent, refactored
+    for every Newton iteration; FP64 assembly, residual, update and epsilon
+    test) for --mixed-steps steps; reports the per-step relative state error
+    between the two and both throughputs.  Every step is timed (the first
+    includes the sweep autotuner, well under a second at this size)."""
     import mafem
     from mafem.analysis import lateral_pull_element, move_tether
     from mafem.mesh import tether_ids
@@ -370,7 +375,7 @@ def bench_precision(args):
     dt = 1e-3
     total = 500
     out = {}
-    for mode in ("mixed", "fp64"):
+    for mode, nsteps in (("fp64", args.steps), ("mixed", min(args.steps, args.mixed_steps))):
         p, mesh, xms, cps = build_case(n, args.motion, sigma=0.0)
         dev = mafem.DeviceMesh(mesh, p)
         if mode == "mixed":
@@ -382,7 +387,7 @@ def bench_precision(args):
         el = lateral_pull_element(mesh, 0, total)
         _, dirs = tether_ids(mesh, p, el, 0.0)
         states, iters, times, stats, els = [], [], [], [], []
-        for k in range(args.steps + 1):
+        for k in range(nsteps):
             new = lateral_pull_element(mesh, k, total)
             if new != el:
                 _, dirs = move_tether(dev, mesh, p, el, new, pull_speed(k))
@@ -397,48 +402,50 @@ def bench_precision(args):
                 raise RuntimeError(f"{mode}: Newton did not converge at step {k}: {eps}")
             st = dev.stats()
             print(f"bench: {mode} step {k} (element {el}): {te:.2f} s, {len(eps)} Newton "
-                  f"iterations, factorizations {st['n_factor']}, gmres {st['gmres_iters']}",
-                  file=sys.stderr, flush=True)
+                  f"iterations, factorizations {st['n_factor']}, gmres {st['gmres_iters']}, "
+                  f"perturbed pivots {st['pivot_perturbations']}", file=sys.stderr, flush=True)
             states.append(dev.get_state())
             els.append(el)
-            if k == 0:
-                continue
             times.append(te)
             iters.append(len(eps))
             stats.append(st)
         out[mode] = (states, iters, times, stats, mesh, els)
         dev.close()
-    sm, im, tm, stm, mesh, els = out["mixed"]
-    sf, iF, tf, stf, _, _ = out["fp64"]
+    sf, iF, tf, stf, mesh, els = out["fp64"]
+    sm, im, tm, stm, _, _ = out["mixed"]
     errs = [max(np.linalg.norm(a[0] - b[0]) / np.linalg.norm(b[0]),
                 np.linalg.norm(a[1] - b[1]) / np.linalg.norm(b[1])) for a, b in zip(sm, sf)]
+    fp64_rate = sum(iF) / sum(tf)
+    mixed_rate = sum(im) / sum(tm) if tm else None
     res = {
         "metric": METRIC,
-        "value": sum(im) / sum(tm),
+        "value": fp64_rate,
         "unit": "Newton steps/s",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": 1,
-        "ms_per_step": 1e3 * sum(tm) / args.steps,
+        "warmup": 0,
+        "ms_per_step": 1e3 * sum(tf) / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 (FP32 band-LU preconditioner of the current tangent in FP64 FGMRES)",
+        "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"F_PULL {args.motion} {n}x{n} patch, tether moved across 40% of "
                                f"the width over 500 steps (elements {els[0]}..{els[-1]} here), "
-                               f"flat initial state, pull ramped to 2.0 over 200 steps, dt=1e-3",
+                               f"flat initial state, pull ramped to 2.0 over 200 steps, dt=1e-3; "
+                               f"FP64 direct band LU",
                    "nmdf": mesh.nmdf, "numel": mesh.numel},
-        "newton_iters_mixed": im, "newton_iters_fp64": iF,
-        "per_step_rel_err_vs_fp64": errs,
-        "fp64_direct_newton_per_s": sum(iF) / sum(tf),
-        "mixed_over_fp64": (sum(im) / sum(tm)) / (sum(iF) / sum(tf)),
-        "step_s_mixed": tm, "step_s_fp64": tf,
+        "newton_iters_fp64": iF, "newton_iters_mixed": im,
+        "per_step_rel_err_mixed_vs_fp64": errs,
+        "mixed_newton_per_s": mixed_rate,
+        "mixed_over_fp64": mixed_rate / fp64_rate if mixed_rate else None,
+        "step_s_fp64": tf, "step_s_mixed": tm,
         "gmres_iters_mixed": [s["gmres_iters"] for s in stm],
-        "factor_us_mixed": [s["factor_us"] for s in stm],
+        "pivot_perturbations_mixed": [s["pivot_perturbations"] for s in stm],
         "factor_us_fp64": [s["factor_us"] for s in stf],
-        "solve_us_mixed": [s["solve_us"] for s in stm],
+        "factor_us_mixed": [s["factor_us"] for s in stm],
         "solve_us_fp64": [s["solve_us"] for s in stf],
+        "solve_us_mixed": [s["solve_us"] for s in stm],
     }
     print(json.dumps(res), flush=True)
 
@@ -459,6 +466,8 @@ def main():
     ap.add_argument("--requests", type=int, default=48)
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--req-steps", type=int, default=50)
+    ap.add_argument("--mixed-steps", type=int, default=1,
+                    help="precision workload: steps of the mixed-precision leg")
     ap.add_argument("--max-members", type=int, default=64)
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-graphs", action="store_true",
@@ -619,6 +628,27 @@ def main():
                        "factorization_tflops": flops_fact / (agg["factor_us"] * 1e-6 /
                                                              max(1, agg["n_factor"])) / 1e12
                        if agg["factor_us"] > 0 else None}
+    # the area kernel (residual + all tangent columns of every area element,
+    # FP64 forward mode on the vector ALUs): algorithmic FLOPs of the counted
+    # forward-mode evaluation per element (F_res + F_dir per active direction)
+    # over its measured time per assembly; algorithmic bytes per elem-assembly
+    # (state, r_el, K_el read-modify-write, node indices)
+    from mafem.opcount import assembly_flops, elem_bytes
+    area_us = agg["area_kernel_us"] / max(1, agg["n_assemble"])
+    a_flops = assembly_flops(args.motion, mesh.ID, mesh.IX)
+    a_bytes = elem_bytes(args.motion) * numel
+    a_tf = a_flops / (area_us * 1e-6) / 1e12
+    a_traffic = None
+    if AREA_COUNTERS_KIB.get(args.motion) is not None:
+        fk, wk = AREA_COUNTERS_KIB[args.motion]
+        a_traffic = (fk * (262144.0 / 131088.375) + wk * 1.0) * 1024.0
+    res["area_roofline"] = {"bound": "valu", "achieved": a_tf, "peak": FP64_PEAK_TFLOPS,
+                            "unit": "TFLOP/s", "frac": a_tf / FP64_PEAK_TFLOPS,
+                            "flops_per_launch": a_flops, "avg_us": area_us,
+                            "algorithmic_bytes": a_bytes, "traffic": a_traffic,
+                            "achieved_gbs": a_bytes / (area_us * 1e-6) / 1e9,
+                            "kernel": f"k_area ({args.motion} residual + tangent columns of "
+                                      f"{numel} area elements, FP64 forward mode)"}
     if not args.no_cpu_baseline:
         res["cpu_baseline"], res["vs_oracle"] = cpu_baseline(p, mesh, xms, cps, args.cpu_threads,
                                                              dev)
