@@ -55,8 +55,7 @@ def pull_speed(k, total=200, final=2.0):
 # WRITE_SIZE in KiB, mean over one time step's dispatches of the 64x64 ALEVB
 # bench patch), by kernel; see the roofline below.  None: not measured.
 STEP_COUNTERS_KIB = {
-    "k_update_diag": (8914.95051125919, 9064.25),
-    "k_update_diag2": None,
+    "k_step_la": None,
 }
 # the same counters per dispatch of the area kernel of the bench patch, by motion
 AREA_COUNTERS_KIB = {
@@ -583,46 +582,47 @@ def main():
         "factorizations": agg["n_factor"], "gmres_iters": agg["gmres_iters"],
         "band_solves": agg["n_band_solve"],
     }
-    # dominant kernel of the direct FP64 path: k_update_diag, one launch per
-    # tile row of every factorization: the trailing update of step K (tb x tb
-    # tile products A_IJ -= L_IK U_KJ on the matrix cores, 2 * 64^3 FLOP each)
-    # with the 64x64 diagonal LU of step K + 1 (2/3 * 64^3 FLOP) fused in;
-    # the timed launch is the middle step, where all tb^2 tiles are in range.
+    # dominant kernel of the direct FP64 path: k_step_la, one launch per tile
+    # step K of every chain of every factorization: the trailing update of
+    # step K (tb x tb tile products A_IJ -= L_IK U_KJ on the matrix cores,
+    # 2 * 64^3 FLOP each), the 64x64 diagonal LU of step K + 1 (2/3 * 64^3)
+    # and its panel (2 tb triangular solves with 64 right-hand sides, 64^3
+    # each); the timed launch is a middle step, where every tile is in range.
+    # The two-ended factorization advances both chains (the top part downward,
+    # the bottom part upward) in each launch; the separator's tb steps follow
+    # as single-chain launches.
     _, tb = dev_band_shape(dev)
     chains = st["factor_chains"]
     if chains == 2:
-        # two-ended factorization: one launch advances both chains (the top
-        # part downward, the bottom part upward) by one tile step, each a full
-        # tb x tb window plus the next diagonal LU; the separator's tb steps
-        # follow as single-chain launches
         a = (nt - tb) // 2
         launches = max(a, nt - tb - a) + tb
     else:
         launches = nt
     n_tiles = tb * tb
-    flops_step = chains * (n_tiles * 2.0 * TILE ** 3 + 2.0 / 3.0 * TILE ** 3)
+    flops_step = chains * (n_tiles * 2.0 * TILE ** 3 + 2.0 / 3.0 * TILE ** 3 + 2 * tb * TILE ** 3)
     s_us = float(np.mean(step_us)) if step_us else float("nan")
     achieved = flops_step / (s_us * 1e-6) / 1e12
-    # HBM bytes per k_update_diag launch, from two rocprofv3 --pmc passes over
+    # HBM bytes per k_step_la launch, from two rocprofv3 --pmc passes over
     # `bench.py --steps 1 --warmup 0 --no-cpu-baseline` (FETCH_SIZE, then
-    # WRITE_SIZE, KiB per dispatch averaged over its k_update_diag dispatches),
+    # WRITE_SIZE, KiB per dispatch averaged over its k_step_la dispatches),
     # each scaled by the same counter's reading for the 256 MiB copy of
     # `bench.py --copy-probe` (FETCH_SIZE 131088.375 KiB, WRITE_SIZE 262144 KiB
-    # for 262144 KiB moved); profiles/k_update_diag_hbm_traffic_pmc.txt holds them.
+    # for 262144 KiB moved).
     traffic = None
-    counters = STEP_COUNTERS_KIB["k_update_diag2" if chains == 2 else "k_update_diag"]
+    counters = STEP_COUNTERS_KIB["k_step_la"]
     if counters is not None:
         traffic = (counters[0] * (262144.0 / 131088.375) + counters[1] * 1.0) * 1024.0
     s_bytes = TILE * TILE * 8
+    # algorithmic tiles moved per chain: the window read and written (2 tb^2),
+    # step K's L and U panel tiles read (2 tb), the two next-panel tiles outside
+    # the window read and written (4), the L tiles of row K+1 permuted (2 tb)
     res["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                        "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                        "traffic": traffic, "traffic_unit": "bytes per launch",
-                       "algorithmic_bytes": chains * (2 * n_tiles + 2 * tb) * s_bytes,
-                       "kernel": ("k_update_diag2 (one tile step of both chains of the two-ended "
-                                  "band LU: trailing updates on the matrix cores, each chain's "
-                                  "next 64x64 diagonal LU fused in)" if chains == 2 else
-                                  "k_update_diag (trailing update of one factorization step on "
-                                  "the matrix cores, next 64x64 diagonal LU fused in)"),
+                       "algorithmic_bytes": chains * (2 * n_tiles + 4 * tb + 4) * s_bytes,
+                       "kernel": ("k_step_la (one tile step of each chain of the band LU: trailing "
+                                  "update on the matrix cores, the next 64x64 diagonal LU and "
+                                  "the next panel in the same launch)"),
                        "flops_per_launch": flops_step, "avg_us": s_us,
                        "launches_per_factorization": launches,
                        "factorization_tflops": flops_fact / (agg["factor_us"] * 1e-6 /

@@ -58,7 +58,7 @@ void free_all(Ctx &c) {
                   c.d_mdot_part, c.d_t2, c.d_presc_idx, c.d_presc_val, c.d_s1, c.d_s2,
                   c.d_sweep_err, c.d_pf_el, c.d_pf_slot, c.d_pf_rel, c.d_pf_out,
                   c.d_obel, c.d_crnr, c.d_out, c.d_amax, c.d_tw_z, c.d_tw_br,
-                  c.d_tw_cr, c.d_tw_dr, c.d_tw_s1, c.d_tw_s2};
+                  c.d_tw_cr, c.d_tw_dr, c.d_tw_s1, c.d_tw_s2, c.d_pub};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c.h_red) (void)hipHostFree(c.h_red);
@@ -451,23 +451,24 @@ void build(Ctx &c, const mafem_mesh_desc *m, const mafem_params *p) {
   c.d_s1 = dalloc<double>(c, nv);
   c.d_s2 = dalloc<double>(c, nv);
   c.d_sweep_err = dalloc<int>(c, 1);
+  c.d_pub = dalloc<unsigned long long>(c, 4 * ((int64_t)TILE * TILE + TILE));
   c.d_amax = dalloc<unsigned long long>(c, 2);
   MAFEM_HIP(hipMemset(c.d_amax, 0, 2 * sizeof(unsigned long long)));
   MAFEM_HIP(hipMemset(c.d_sweep_err, 0, sizeof(int)));
 }
 
-// A multi-workgroup sweep whose waits ran out leaves an error count (its
+// A wait between workgroups of one launch (multi-workgroup sweeps, the
+// factorization's look-ahead panels) that ran out leaves an error count (the
 // results are invalid): checked at the end of every solve and time step.
 void check_sweeps(Ctx &c) {
-  if (c.sweep_groups <= 1) return;
   int e = 0;
   MAFEM_HIP(hipMemcpyAsync(&e, c.d_sweep_err, sizeof(int), hipMemcpyDeviceToHost, c.stream));
   MAFEM_HIP(hipStreamSynchronize(c.stream));
   if (e != 0) {
     MAFEM_HIP(hipMemsetAsync(c.d_sweep_err, 0, sizeof(int), c.stream));
-    throw ArgError{"a triangular sweep wait ran out: the solves since the last check are "
-                   "invalid and so is the context state they updated (set it again "
-                   "before continuing)", MAFEM_EHIP};
+    throw ArgError{"a wait inside a factorization or triangular sweep ran out: the solves "
+                   "since the last check are invalid and so is the context state they "
+                   "updated (set it again before continuing)", MAFEM_EHIP};
   }
 }
 

@@ -6,7 +6,8 @@
 // half bandwidth kl = (2 num1np + 3) ndf - 1 and LU without pivoting across
 // tile rows keeps the fill inside that band.  Partial pivoting is applied
 // inside each diagonal tile.  Per tile step K:
-//   k_diag   : P_K A_KK = L_KK U_KK in one wavefront (rows in registers);
+//   k_diag_mm: P_K A_KK = L_KK U_KK by one block (pivot steps in one
+//              wavefront, rank-16 trailing updates on the matrix cores);
 //   k_panel  : U_KJ = L_KK^-1 P_K A_KJ, L_IK = A_IK U_KK^-1 (triangular
 //              solves, one wavefront per tile) and P_K applied to the L tiles
 //              already stored in tile row K;
@@ -27,6 +28,29 @@ constexpr int T = TILE;
 template <class S>
 __device__ __forceinline__ S *tile_ptr(S *band, int tb, int64_t I, int64_t J) {
   return band + ((I * (2 * tb + 1) + (J - I + tb)) * (int64_t)T * T);
+}
+
+// Values published between workgroups of one launch (triangular sweeps,
+// factorization look-ahead) go through words pre-filled with this pattern (a
+// signalling NaN that no result takes), written and read with agent-scope
+// atomics only; a word is its own ready flag.  Waits are bounded by SPIN_LIMIT.
+constexpr unsigned long long SENTINEL = 0x7FF4DEADBEEF1234ULL;
+constexpr long SPIN_LIMIT = 20000000L;
+constexpr int PUBW = T * T + T;  // look-ahead staging: a factored tile, then its pivots
+
+template <class S> __device__ __forceinline__ unsigned long long pub_bits(S x);
+template <> __device__ __forceinline__ unsigned long long pub_bits<double>(double x) {
+  return (unsigned long long)__double_as_longlong(x);
+}
+template <> __device__ __forceinline__ unsigned long long pub_bits<float>(float x) {
+  return (unsigned long long)__float_as_uint(x);
+}
+template <class S> __device__ __forceinline__ S pub_val(unsigned long long v);
+template <> __device__ __forceinline__ double pub_val<double>(unsigned long long v) {
+  return __longlong_as_double((long long)v);
+}
+template <> __device__ __forceinline__ float pub_val<float>(unsigned long long v) {
+  return __uint_as_float((unsigned)v);
 }
 
 // ----------------------------------------------------------------- band fill
@@ -209,132 +233,6 @@ __device__ __forceinline__ float lane_bcast(float x, int l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
 }
 
-// LU with partial pivoting of the 64x64 diagonal tile in one wavefront: lane
-// i holds row i in registers (staged through LDS so the global accesses are
-// coalesced).  Rows are not moved: the candidate row with the largest |a_ij|
-// (pivot_key) is the pivot of step j, and the row chosen at step j is stored
-// as row j of the factored tile (P A = L U); piv[K][j] records its original
-// index.  Blocked in four panels of 16 columns: the 16 pivot steps of a panel
-// eliminate only inside the panel (the pivot row's entries broadcast with
-// readlane, non-candidates using a zero multiplier), then the panel's pivot
-// rows publish their trailing entries and multipliers to LDS, U12 = L11^-1
-// A12 is solved with one lane per trailing column, and every remaining
-// candidate row takes the rank-16 update A22 -= L21 U12 from broadcast LDS
-// reads.  Same arithmetic as the unblocked elimination, with the per-step
-// chain shortened from 64 - j to at most 15 columns.
-// The tile is already in st (st[i][0..63] = row i); only wave 0 of the
-// block works, other waves (when the block is larger) take part in the
-// barriers only.  The factored tile goes to A, the pivots to piv[K].
-// amax (used when PERT): static pivoting.  amax[0] holds max |K| as the bits of a
-// non-negative double; a pivot of magnitude below thr = eps max |K| (eps of
-// the storage precision: numerically zero) is replaced by +-thr (the
-// factorization is then that of a nearby matrix, and the solve finishes with
-// FGMRES) and counted in amax[1].
-template <class S, bool PERT = false>
-__device__ __forceinline__ void diag_core(int64_t K, S *__restrict__ A, int32_t *__restrict__ piv,
-                                          S (*st)[T + 1], S (*u12)[T], S (*l11)[16],
-                                          unsigned long long *amax = nullptr) {
-  const bool on = threadIdx.x < 64;
-  const int lane = (int)(threadIdx.x & 63);
-  S thr = S(0);
-  if (PERT) {
-    const double rel = sizeof(S) == 8 ? 2.220446049250313e-16 : 1.1920928955078125e-7;
-    thr = (S)(rel * __longlong_as_double((long long)__hip_atomic_load(
-                           amax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
-  }
-  int npert = 0;
-  S r[T];
-#pragma unroll
-  for (int k = 0; k < T; ++k) r[k] = on ? st[lane][k] : S(0);
-  int pos = -1;
-#pragma unroll
-  for (int pb = 0; pb < 4; ++pb) {
-    const int c0 = pb * 16;
-#pragma unroll
-    for (int jj = 0; jj < 16; ++jj) {
-      const int j = c0 + jj;
-      const unsigned key = wave_max_u32(pivot_key(r[j], pos < 0, lane));
-      const int p = 63 - (int)(key & 63u);
-      S pj = lane_bcast(r[j], p);
-      bool tiny = false;
-      if (PERT) {  // wave-uniform
-        tiny = fabs(pj) < thr;
-        pj = tiny ? (pj < S(0) ? -thr : thr) : pj;
-        npert += tiny ? 1 : 0;
-      }
-      if (lane == p) pos = j;
-      const S l = (pos < 0) ? r[j] / pj : S(0);
-      if (pos < 0) r[j] = l;
-      if (PERT && tiny && lane == p) r[j] = pj;
-#pragma unroll
-      for (int k = j + 1; k < c0 + 16; ++k) r[k] = fma(-l, lane_bcast(r[k], p), r[k]);
-    }
-    if (pb == 3) break;
-    const bool mine = pos >= c0 && pos < c0 + 16;  // pivot row of this panel
-    if (on && mine) {
-      const int i = pos - c0;
-#pragma unroll
-      for (int k = c0 + 16; k < T; ++k) u12[i][k] = r[k];
-#pragma unroll
-      for (int jj = 0; jj < 16; ++jj) l11[i][jj] = (jj < i) ? r[c0 + jj] : S(0);
-    }
-    __syncthreads();
-    if (on && lane >= c0 + 16) {  // U12 = L11^-1 A12, one trailing column per lane
-      S u[16];
-#pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        S acc = u12[i][lane];
-#pragma unroll
-        for (int ii = 0; ii < i; ++ii) acc = fma(-l11[i][ii], u[ii], acc);
-        u[i] = acc;
-      }
-#pragma unroll
-      for (int i = 0; i < 16; ++i) u12[i][lane] = u[i];
-    }
-    __syncthreads();
-    if (on && pos < 0) {  // A22 -= L21 U12
-#pragma unroll
-      for (int k = c0 + 16; k < T; ++k) {
-        S acc = r[k];
-#pragma unroll
-        for (int jj = 0; jj < 16; ++jj) acc = fma(-r[c0 + jj], u12[jj][k], acc);
-        r[k] = acc;
-      }
-    } else if (on && mine) {
-#pragma unroll
-      for (int k = c0 + 16; k < T; ++k) r[k] = u12[pos - c0][k];
-    }
-    __syncthreads();
-  }
-  __syncthreads();
-  if (on) {
-#pragma unroll
-    for (int k = 0; k < T; ++k) st[pos][k] = r[k];
-    piv[K * T + pos] = lane;
-    if (PERT && npert > 0 && lane == 0) atomicAdd(amax + 1, (unsigned long long)npert);
-  }
-  __syncthreads();
-  if (on) {
-#pragma unroll 8
-    for (int i = 0; i < T; ++i) A[i * T + lane] = st[i][lane];
-  }
-}
-
-template <class S, bool PERT = false>
-__device__ __forceinline__ void diag_body(int64_t K, int tb, S *__restrict__ band,
-                                          int32_t *__restrict__ piv,
-                                          unsigned long long *amax = nullptr) {
-  __shared__ S st[T][T + 1];
-  __shared__ S u12[16][T];
-  __shared__ S l11[16][16];
-  const int lane = threadIdx.x;
-  S *A = tile_ptr(band, tb, K, K);
-#pragma unroll 8
-  for (int i = 0; i < T; ++i) st[i][lane] = A[i * T + lane];
-  __syncthreads();
-  diag_core<S, PERT>(K, A, piv, st, u12, l11, amax);
-}
-
 // amax[0] = max |a| over n values (non-negative doubles order like their bits)
 template <class S>
 __global__ __launch_bounds__(256) void k_absmax(int64_t n, const S *__restrict__ a,
@@ -352,49 +250,22 @@ __global__ void k_zero_u64(unsigned long long *__restrict__ p) {
   if (threadIdx.x == 0) p[0] = 0;
 }
 
-// Panel of step K, one wavefront per tile.  kind 0: U_KJ = L_KK^-1 P_K A_KJ
-// (lane = column, forward substitution); kind 1: L_IK = A_IK U_KK^-1 (lane =
-// row, substitution against U); kind 2: P_K applied to the L tiles already
-// stored left of the diagonal in tile row K.  The factored diagonal tile is
-// staged in LDS and read with wave-uniform (broadcast) addresses; the tile
-// being solved lives in LDS and is processed in blocks of 8 unknowns by loops
-// whose bodies are small (a fully unrolled 64 x 64 substitution is ~9 000
+// Triangular solves of a panel tile by one wavefront, both tiles in LDS (the
+// factors read with wave-uniform addresses, rows unpadded so that they load
+// 16 bytes at a time; the tile's rows padded against bank conflicts):
+// kind 0, X <- L^-1 X (unit lower L of the factored diagonal tile D; lane =
+// column, forward substitution); kind 1, X <- X U^-1 (lane = row, rdiag[k] =
+// 1 / D_kk).  The tile is processed in blocks of 8 unknowns by loops whose
+// bodies are small (a fully unrolled 64 x 64 substitution is ~9 000
 // straight-line instructions, and fetching them dominated the kernel).  Every
 // entry receives its updates in increasing order of the eliminated index, as
-// in the unblocked substitution, so the results are the same bit for bit.
+// in the unblocked substitution.
 template <class S>
-__device__ __forceinline__ void panel_body(int64_t bid, int64_t K, int64_t nt, int tb, S *__restrict__ band,
-                                              const int32_t *__restrict__ piv) {
+__device__ __forceinline__ void trsm_wave(int kind, const S (*sD)[T], S (*sX)[T + 1],
+                                          const S *rdiag) {
   constexpr int B = 8;
-  __shared__ S sD[T][T];
-  __shared__ S sX[T][T + 1];
-  __shared__ S rdiag[T];
-  const int lane = threadIdx.x;
-  const int kind = bid / tb, off = bid % tb;
-  const int32_t *pv = piv + K * T;
-  if (kind == 2) {
-    const int64_t J = K - tb + off;
-    if (J < 0) return;
-    S *A = tile_ptr(band, tb, K, J);
-    S x[T];
-#pragma unroll
-    for (int i = 0; i < T; ++i) x[i] = A[(int64_t)pv[i] * T + lane];
-#pragma unroll
-    for (int i = 0; i < T; ++i) A[i * T + lane] = x[i];
-    return;
-  }
-  const int64_t IJ = K + 1 + off;
-  if (IJ >= nt) return;
-  const S *D = tile_ptr(band, tb, K, K);
-  // full unrolling keeps all 64 row loads of a tile in flight (one wave per
-  // tile: grouped loads would pay the memory latency once per group)
-#pragma unroll
-  for (int i = 0; i < T; ++i) sD[i][lane] = D[i * T + lane];
+  const int lane = (int)(threadIdx.x & 63);
   if (kind == 0) {
-    S *A = tile_ptr(band, tb, K, IJ);
-#pragma unroll
-    for (int i = 0; i < T; ++i) sX[i][lane] = A[(int64_t)pv[i] * T + lane];
-    __syncthreads();
     // x_i -= D_ik x_k for k < i, k increasing (unit lower L)
 #pragma unroll 1
     for (int b = 0; b < T / B; ++b) {
@@ -418,15 +289,7 @@ __device__ __forceinline__ void panel_body(int64_t bid, int64_t K, int64_t nt, i
 #pragma unroll
       for (int r = 0; r < B; ++r) sX[B * b + r][lane] = xb[r];
     }
-#pragma unroll 8
-    for (int i = 0; i < T; ++i) A[i * T + lane] = sX[i][lane];
   } else {
-    S *A = tile_ptr(band, tb, IJ, K);
-#pragma unroll
-    for (int i = 0; i < T; ++i) sX[i][lane] = A[i * T + lane];
-    __syncthreads();
-    rdiag[lane] = S(1) / sD[lane][lane];
-    __syncthreads();
     // row `lane`: x_k -= x_j D_jk for j < k, j increasing; x_k *= 1 / D_kk
 #pragma unroll 1
     for (int b = 0; b < T / B; ++b) {
@@ -452,10 +315,53 @@ __device__ __forceinline__ void panel_body(int64_t bid, int64_t K, int64_t nt, i
 #pragma unroll
       for (int r = 0; r < B; ++r) sX[lane][B * b + r] = xb[r];
     }
-    __syncthreads();
-#pragma unroll 8
-    for (int i = 0; i < T; ++i) A[i * T + lane] = sX[i][lane];
   }
+}
+
+// Panel of step K, one wavefront per tile.  kind 0: U_KJ = L_KK^-1 P_K A_KJ
+// (lane = column); kind 1: L_IK = A_IK U_KK^-1 (lane = row); kind 2: P_K
+// applied to the L tiles already stored left of the diagonal in tile row K.
+// The factored diagonal tile is staged in LDS and read with wave-uniform
+// (broadcast) addresses.
+template <class S>
+__device__ __forceinline__ void panel_body(int64_t bid, int64_t K, int64_t nt, int tb, S *__restrict__ band,
+                                              const int32_t *__restrict__ piv) {
+  __shared__ __align__(16) S sD[T][T];
+  __shared__ S sX[T][T + 1];
+  __shared__ S rdiag[T];
+  const int lane = threadIdx.x;
+  const int kind = bid / tb, off = bid % tb;
+  const int32_t *pv = piv + K * T;
+  if (kind == 2) {
+    const int64_t J = K - tb + off;
+    if (J < 0) return;
+    S *A = tile_ptr(band, tb, K, J);
+    S x[T];
+#pragma unroll
+    for (int i = 0; i < T; ++i) x[i] = A[(int64_t)pv[i] * T + lane];
+#pragma unroll
+    for (int i = 0; i < T; ++i) A[i * T + lane] = x[i];
+    return;
+  }
+  const int64_t IJ = K + 1 + off;
+  if (IJ >= nt) return;
+  const S *D = tile_ptr(band, tb, K, K);
+  // full unrolling keeps all 64 row loads of a tile in flight (one wave per
+  // tile: grouped loads would pay the memory latency once per group)
+#pragma unroll
+  for (int i = 0; i < T; ++i) sD[i][lane] = D[i * T + lane];
+  S *A = kind == 0 ? tile_ptr(band, tb, K, IJ) : tile_ptr(band, tb, IJ, K);
+#pragma unroll
+  for (int i = 0; i < T; ++i) sX[i][lane] = A[(int64_t)(kind == 0 ? pv[i] : i) * T + lane];
+  __syncthreads();
+  if (kind == 1) {
+    rdiag[lane] = S(1) / sD[lane][lane];
+    __syncthreads();
+  }
+  trsm_wave<S>(kind, sD, sX, rdiag);
+  __syncthreads();
+#pragma unroll 8
+  for (int i = 0; i < T; ++i) A[i * T + lane] = sX[i][lane];
 }
 
 template <class S>
@@ -584,22 +490,30 @@ __device__ __forceinline__ int acc_row(int j) {
 }
 
 // LU with partial pivoting of the 64x64 diagonal tile by a 256-thread block.
-// The tile is in st (st[i][0..63] = row i, original order).  Panels of 16
-// columns: the first wavefront holds the panel's entries of every row in
-// registers (lane = row) and runs its 16 pivot steps exactly as diag_core
-// does (rows are not moved; the row chosen at step j becomes row j of the
-// factors); the panel's pivot rows then solve U12 = L11^-1 A12 (one trailing
-// column per lane), and the four wavefronts subtract L21 U12 from the rows
-// still unchosen on the matrix cores (wavefront w: rows 16w..16w+15; chosen
-// rows enter with a zero multiplier and are not written).  Only the pivot
-// steps stay on one wavefront.  The factored tile goes to A in pivot order,
-// the pivots to piv[K]; prow / cand: 64 ints of LDS each (the row chosen at
-// step j; whether row i is still a candidate).  amax / PERT: static
-// pivoting as in diag_core.
+// The tile is in st (st[i][0..63] = row i, original order).  Rows are not
+// moved: the candidate row with the largest |a_ij| (pivot_key) is the pivot
+// of step j, and the row chosen at step j is stored as row j of the factored
+// tile (P A = L U).  Panels of 16 columns: the first wavefront holds the
+// panel's entries of every row in registers (lane = row) and runs its 16
+// pivot steps (the pivot row's entries broadcast with readlane; rows no
+// longer candidates take a zero multiplier); the panel's pivot rows then
+// solve U12 = L11^-1 A12 (one trailing column per lane), and the four
+// wavefronts subtract L21 U12 from the rows still unchosen on the matrix cores
+// (wavefront w: rows 16w..16w+15; chosen rows enter with a zero multiplier
+// and are not written).  Only the pivot steps stay on one wavefront.  The
+// factored tile goes to A in pivot order and piv[K][j] records the original
+// index of row j; prow / cand: 64 ints of LDS each (the row chosen at step j;
+// whether row i is still a candidate).
+// amax (used when PERT): static pivoting.  amax[0] holds max |K| as the bits
+// of a non-negative double; a pivot of magnitude below thr = eps max |K| (eps
+// of the storage precision: numerically zero) is replaced by +-thr (the
+// factorization is then that of a nearby matrix, and the solve finishes with
+// FGMRES) and counted in amax[1].
 template <class S, bool PERT = false>
 __device__ __forceinline__ void diag_core_mm(int64_t K, S *__restrict__ A, int32_t *__restrict__ piv,
                                              S (*st)[T + 1], int *prow, int *cand,
-                                             unsigned long long *amax = nullptr) {
+                                             unsigned long long *amax = nullptr,
+                                             unsigned long long *pub = nullptr) {
   using V = typename Mfma<S>::V;
   const int t = (int)threadIdx.x, lane = t & 63, w = t >> 6;
   const int g = lane >> 4, cc = lane & 15;
@@ -691,6 +605,17 @@ __device__ __forceinline__ void diag_core_mm(int64_t K, S *__restrict__ A, int32
     piv[K * T + pos] = lane;
     if (PERT && npert > 0 && lane == 0) atomicAdd(amax + 1, (unsigned long long)npert);
   }
+  if (pub) {  // the factors and the pivots for the look-ahead panel first (PUBW words)
+#pragma unroll
+    for (int it = 0; it < T / 4; ++it) {
+      const int i = 4 * it + w;
+      __hip_atomic_store(&pub[i * T + lane], pub_bits<S>(st[prow[i]][lane]), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (w == 0)
+      __hip_atomic_store(&pub[T * T + lane], (unsigned long long)prow[lane], __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+  }
 #pragma unroll
   for (int it = 0; it < T / 4; ++it) {
     const int i = 4 * it + w;
@@ -702,18 +627,18 @@ __device__ __forceinline__ void diag_core_mm(int64_t K, S *__restrict__ A, int32
 // buffer of MM_LDS entries (its index arrays follow the tile)
 template <class S, bool PERT>
 __device__ __forceinline__ void diag_mm_lds(int64_t K, S *__restrict__ A, int32_t *__restrict__ piv,
-                                            S *lds, unsigned long long *amax) {
+                                            S *lds, unsigned long long *amax,
+                                            unsigned long long *pub = nullptr) {
   S(*st)[T + 1] = reinterpret_cast<S(*)[T + 1]>(lds);
   int *prow = reinterpret_cast<int *>(lds + T * (T + 1));
-  diag_core_mm<S, PERT>(K, A, piv, st, prow, prow + T, amax);
+  diag_core_mm<S, PERT>(K, A, piv, st, prow, prow + T, amax, pub);
 }
 
-// the first diagonal LU of a chain: tile (K, K) of the band, 256 threads
+// diagonal LU of tile (K, K) of a band by a 256-thread block
 template <class S, bool PERT>
-__global__ __launch_bounds__(256) void k_diag_mm(int64_t K, int tb, S *__restrict__ band,
-                                                 int32_t *__restrict__ piv,
-                                                 unsigned long long *__restrict__ amax) {
-  __shared__ __align__(16) S lds[MM_LDS];
+__device__ __forceinline__ void diag_block(int64_t K, int tb, S *__restrict__ band,
+                                           int32_t *__restrict__ piv, unsigned long long *amax,
+                                           S *lds) {
   S *A = tile_ptr(band, tb, K, K);
   const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
 #pragma unroll
@@ -725,25 +650,13 @@ __global__ __launch_bounds__(256) void k_diag_mm(int64_t K, int tb, S *__restric
   diag_mm_lds<S, PERT>(K, A, piv, lds, amax);
 }
 
-template <class S>
-__global__ __launch_bounds__(256) void k_update(int64_t K, int64_t nt, int tb, S *__restrict__ band) {
-  __shared__ __align__(16) S lds[MM_LDS];
-  update_body<S>((int64_t)blockIdx.x, K, nt, tb, band, lds);
-}
-
-// Trailing update of step K with the next diagonal LU fused in: block 0
-// updates tile (K+1, K+1) into LDS, then its first wavefront factors it
-// while the other blocks are still updating (requires K + 1 < nt).
+// the first diagonal LU of a chain
 template <class S, bool PERT>
-__global__ __launch_bounds__(256) void k_update_diag(int64_t K, int64_t nt, int tb,
-                                                     S *__restrict__ band, int32_t *__restrict__ piv,
-                                                     unsigned long long *__restrict__ amax) {
+__global__ __launch_bounds__(256) void k_diag_mm(int64_t K, int tb, S *__restrict__ band,
+                                                 int32_t *__restrict__ piv,
+                                                 unsigned long long *__restrict__ amax) {
   __shared__ __align__(16) S lds[MM_LDS];
-  const bool fused = blockIdx.x == 0;
-  update_body<S>((int64_t)blockIdx.x, K, nt, tb, band, lds, fused);
-  if (!fused) return;
-  __syncthreads();
-  diag_mm_lds<S, PERT>(K + 1, tile_ptr(band, tb, K + 1, K + 1), piv, lds, amax);
+  diag_block<S, PERT>(K, tb, band, piv, amax, lds);
 }
 
 // Both chains of the two-ended factorization in one launch (blocks of chain 1
@@ -754,6 +667,7 @@ struct FacChain {
   int32_t *piv;
   int64_t K, nt;
   int32_t fuse;
+  unsigned long long *pub;  // look-ahead staging, 2 PUBW words
 };
 
 template <class S>
@@ -765,21 +679,163 @@ __global__ __launch_bounds__(64) void k_panel2(int tb, FacChain c0, FacChain c1)
   panel_body<S>((int64_t)blockIdx.x - (second ? nb0 : 0), ch.K, ch.nt, tb, (S *)ch.band, ch.piv);
 }
 
+// ------------------------------------- factorization step with look-ahead
+//
+// One launch per tile step K of a chain does the trailing update of step K,
+// the diagonal LU of step K + 1 and the panel of step K + 1, so the next
+// step needs no separate panel launch:
+//   block 0          : tile (K+1, K+1) updated and factored (diag_core_mm),
+//                      its factors and pivots published to a staging buffer;
+//   blocks 1 .. tb   : U panel tiles (K+1, J), J = K+2 .. K+1+tb: updated by
+//                      step K when inside its window, then, once the factors of
+//                      K + 1 are published, L^-1 P X (one wavefront);
+//   tb+1 .. 2 tb     : L panel tiles (I, K+1) likewise, X U^-1;
+//   2 tb+1 .. 3 tb   : P_(K+1) applied to the L tiles of tile row K+1;
+//   the rest         : the other (tb - 1)^2 tiles of step K's window.
+// Two staging buffers alternate with the step's parity; block 1 refills the
+// one read in the previous launch with the sentinel.  Every waiting block
+// waits only for block 0 of its chain, which is dispatched before it, so the
+// waits end; each is bounded (on expiry the error count is raised and the
+// block goes on with an invalid result).  A chain with fuse = 0 (its next
+// diagonal tile is not complete) takes the plain trailing update.
+constexpr int step_blocks(int tb) { return tb * tb + tb + 2; }
+
+// waits until the staging words are published and copies them: the factored
+// tile to sD (when set), the pivots to spiv.  One lane watches the last word
+// first; then every thread issues all its loads at once and re-reads only
+// words still holding the sentinel.
+template <class S>
+__device__ __forceinline__ void pub_wait(const unsigned long long *__restrict__ pub, S (*sD)[T],
+                                         int *spiv, int *__restrict__ err, int *bad) {
+  constexpr int NW = (PUBW + 255) / 256;
+  if (threadIdx.x == 0) {
+    *bad = 0;
+    long spins = 0;
+    while (__hip_atomic_load(&pub[PUBW - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+           SENTINEL) {
+      __builtin_amdgcn_s_sleep(2);
+      if (++spins > SPIN_LIMIT) {
+        *bad = 1;
+        atomicAdd(err, 1);
+        break;
+      }
+    }
+  }
+  __syncthreads();
+  if (*bad) return;
+  const int i0 = (sD ? 0 : T * T) + (int)threadIdx.x;
+  unsigned long long v[NW];
+#pragma unroll
+  for (int q = 0; q < NW; ++q) {
+    const int idx = i0 + 256 * q;
+    v[q] = idx < PUBW ? __hip_atomic_load(&pub[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                      : 0ull;
+  }
+#pragma unroll
+  for (int q = 0; q < NW; ++q) {
+    const int idx = i0 + 256 * q;
+    if (idx >= PUBW) continue;
+    long spins = 0;
+    while (v[q] == SENTINEL) {
+      __builtin_amdgcn_s_sleep(1);
+      v[q] = __hip_atomic_load(&pub[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (++spins > SPIN_LIMIT) {
+        atomicAdd(err, 1);
+        break;
+      }
+    }
+    if (idx < T * T)
+      sD[idx / T][idx % T] = pub_val<S>(v[q]);
+    else
+      spiv[idx - T * T] = (int)v[q];
+  }
+  __syncthreads();
+}
+
 template <class S, bool PERT>
-__global__ __launch_bounds__(256) void k_update_diag2(int tb, FacChain c0, FacChain c1,
-                                                      unsigned long long *__restrict__ amax) {
+__global__ __launch_bounds__(256) void k_step_la(int tb, FacChain c0, FacChain c1,
+                                                 unsigned long long *__restrict__ amax,
+                                                 int *__restrict__ err) {
   __shared__ __align__(16) S lds[MM_LDS];
-  const int nb0 = tb * tb;
-  const bool second = (int)blockIdx.x >= nb0;
+  __shared__ S rdiag[T];
+  __shared__ int spiv[T];
+  __shared__ int bad;
+  const int nbc = step_blocks(tb);
+  const bool second = (int)blockIdx.x >= nbc;
   const FacChain &ch = second ? c1 : c0;
   if (ch.nt == 0) return;
-  const int64_t bid = (int64_t)blockIdx.x - (second ? nb0 : 0);
+  const int64_t bid = (int64_t)blockIdx.x - (second ? nbc : 0);
   S *band = (S *)ch.band;
-  const bool fused = bid == 0 && ch.fuse;
-  update_body<S>(bid, ch.K, ch.nt, tb, band, lds, fused);
-  if (!fused) return;
-  __syncthreads();
-  diag_mm_lds<S, PERT>(ch.K + 1, tile_ptr(band, tb, ch.K + 1, ch.K + 1), ch.piv, lds, amax);
+  const int64_t K = ch.K, nt = ch.nt;
+  if (!ch.fuse) {
+    if (bid < (int64_t)tb * tb) update_body<S>(bid, K, nt, tb, band, lds);
+    return;
+  }
+  unsigned long long *pub_new = ch.pub + ((K + 1) & 1) * PUBW;
+  const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+  if (bid == 0) {
+    // the critical path of the launch: its wavefronts issue first on their CU
+    __builtin_amdgcn_s_setprio(3);
+    update_body<S>(0, K, nt, tb, band, lds, true);
+    __syncthreads();
+    diag_mm_lds<S, PERT>(K + 1, tile_ptr(band, tb, K + 1, K + 1), ch.piv, lds, amax, pub_new);
+    return;
+  }
+  const int64_t r = bid - 1;
+  if (r < 3 * (int64_t)tb) {
+    if (r == 0) {  // refill the staging buffer read in the previous launch
+      unsigned long long *pub_old = ch.pub + (K & 1) * PUBW;
+      for (int idx = (int)threadIdx.x; idx < PUBW; idx += 256)
+        __hip_atomic_store(&pub_old[idx], SENTINEL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    const int kind = (int)(r / tb), off = (int)(r % tb);
+    S(*sX)[T + 1] = reinterpret_cast<S(*)[T + 1]>(lds);
+    S(*sD)[T] = reinterpret_cast<S(*)[T]>(lds + T * (T + 1));
+    if (kind == 2) {  // P_(K+1) on the L tile (K+1, J)
+      const int64_t J = K + 1 - tb + off;
+      if (J < 0) return;
+      pub_wait<S>(pub_new, nullptr, spiv, err, &bad);
+      S *A = tile_ptr(band, tb, K + 1, J);
+      S v[T / 4];
+#pragma unroll
+      for (int it = 0; it < T / 4; ++it) v[it] = A[(int64_t)spiv[4 * it + w] * T + lane];
+      __syncthreads();
+#pragma unroll
+      for (int it = 0; it < T / 4; ++it) A[(4 * it + w) * T + lane] = v[it];
+      return;
+    }
+    const int64_t IJ = K + 2 + off;
+    if (IJ >= nt) return;
+    const int64_t I = kind == 0 ? K + 1 : IJ, J = kind == 0 ? IJ : K + 1;
+    S *A = tile_ptr(band, tb, I, J);
+    if (IJ <= K + tb) {  // inside step K's window: A_IJ - L_IK U_KJ, into LDS
+      update_body<S>((I - K - 1) * tb + (J - K - 1), K, nt, tb, band, lds, true);
+    } else {
+#pragma unroll
+      for (int it = 0; it < T / 4; ++it) sX[4 * it + w][lane] = A[(4 * it + w) * T + lane];
+    }
+    pub_wait<S>(pub_new, sD, spiv, err, &bad);
+    if (kind == 0) {  // P_(K+1): row i of X is row spiv[i] of the updated tile
+      S v[T / 4];
+#pragma unroll
+      for (int it = 0; it < T / 4; ++it) v[it] = sX[spiv[4 * it + w]][lane];
+      __syncthreads();
+#pragma unroll
+      for (int it = 0; it < T / 4; ++it) sX[4 * it + w][lane] = v[it];
+    } else if (threadIdx.x < T) {
+      rdiag[threadIdx.x] = S(1) / sD[threadIdx.x][threadIdx.x];
+    }
+    __syncthreads();
+    if (w == 0) trsm_wave<S>(kind, sD, sX, rdiag);
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < T / 4; ++it) A[(4 * it + w) * T + lane] = sX[4 * it + w][lane];
+    return;
+  }
+  const int64_t u = r - 3 * (int64_t)tb, tm = tb - 1;
+  if (u >= tm * tm) return;
+  const int64_t I = K + 2 + u / tm, J = K + 2 + u % tm;
+  update_body<S>((I - K - 1) * tb + (J - K - 1), K, nt, tb, band, lds);
 }
 
 // ------------------------------------------------- post-factorization pass
@@ -1068,8 +1124,6 @@ __global__ __launch_bounds__(SW) void k_fwd_sweep(int64_t nt, int tb, const S *_
 // tile waits for it.  Every wait is bounded: on expiry the kernel counts an
 // error and every workgroup leaves.
 
-constexpr unsigned long long SENTINEL = 0x7FF4DEADBEEF1234ULL;  // a signalling-NaN pattern
-constexpr long SPIN_LIMIT = 20000000L;
 
 // The sweep outputs are written and read only with agent-scope atomics
 // (never through a cached plain access), so no stale cache line can stand in
@@ -1372,18 +1426,19 @@ __device__ __forceinline__ int step_locate(const StepMap &sm, const int32_t *__r
 }
 
 template <class S>
-__global__ __launch_bounds__(64) void kb_diag(StepMap sm, int64_t K, const int32_t *__restrict__ order,
-                                              const PatchDev *__restrict__ pd,
-                                              const uint8_t *__restrict__ on,
-                                              const uint8_t *__restrict__ fac,
-                                              const uint8_t *__restrict__ pert) {
+__global__ __launch_bounds__(256) void kb_diag(StepMap sm, int64_t K, const int32_t *__restrict__ order,
+                                               const PatchDev *__restrict__ pd,
+                                               const uint8_t *__restrict__ on,
+                                               const uint8_t *__restrict__ fac,
+                                               const uint8_t *__restrict__ pert) {
   int local, tb;
   const int p = step_locate<0>(sm, order, local, tb);
   if (!fac_on(on, fac, p) || pd[p].prec != prec_of<S>()) return;
+  __shared__ __align__(16) S lds[MM_LDS];
   if (pert[p])
-    diag_body<S, true>(K, tb, (S *)pd[p].band, pd[p].piv, pd[p].amax);
+    diag_block<S, true>(K, tb, (S *)pd[p].band, pd[p].piv, pd[p].amax, lds);
   else
-    diag_body<S, false>(K, tb, (S *)pd[p].band, pd[p].piv);
+    diag_block<S, false>(K, tb, (S *)pd[p].band, pd[p].piv, nullptr, lds);
 }
 
 // max |K| of every patch of precision S that factors with static pivoting,
@@ -1567,31 +1622,33 @@ static void fill_t(Ctx &c, S *band) {
 }
 
 // Steps k0 .. k1 - 1 of the band LU of a band with nt tile rows on stream s:
-// the diagonal LU of step k0 first (diag_first), then per step the panels
-// and the trailing update with the next diagonal LU fused in (k_update_diag),
-// except after the last step when !fuse_last (its next diagonal tile is not
-// complete yet).  When `timed`, one launch in the middle is bracketed by
-// ev[10] / ev[11].
+// the diagonal LU of step k0 first (diag_first), the panel of step k0, then
+// one look-ahead launch per step (k_step_la: the trailing update, the next
+// diagonal LU and the next panel), except after the last step when
+// !fuse_last (its next diagonal tile is not complete yet: plain update).
+// pub: the chain's 2 PUBW staging words.  When `timed`, one launch in the
+// middle is bracketed by ev[10] / ev[11].
 template <class S, bool PERT>
-static void factor_chain(Ctx &c, hipStream_t s, S *band, int32_t *piv, int64_t nt, int64_t k0,
-                         int64_t k1, bool diag_first, bool fuse_last, bool timed) {
+static void factor_chain(Ctx &c, hipStream_t s, S *band, int32_t *piv, unsigned long long *pub,
+                         int64_t nt, int64_t k0, int64_t k1, bool diag_first, bool fuse_last,
+                         bool timed) {
   if (diag_first)
     hipLaunchKernelGGL((k_diag_mm<S, PERT>), dim3(1), dim3(256), 0, s, k0, c.tb, band, piv, c.d_amax);
+  hipLaunchKernelGGL(k_fill_sentinel, dim3((2 * PUBW + 255) / 256), dim3(256), 0, s,
+                     (int64_t)(2 * PUBW), pub);
+  hipLaunchKernelGGL(k_panel<S>, dim3(3 * c.tb), dim3(64), 0, s, k0, nt, c.tb, band, piv);
   const int64_t timed_K = timed ? (k0 + k1) / 2 : -1;
   for (int64_t K = k0; K < k1; ++K) {
-    hipLaunchKernelGGL(k_panel<S>, dim3(3 * c.tb), dim3(64), 0, s, K, nt, c.tb, band, piv);
     if (K + 1 >= nt) break;  // no trailing tiles after the last step
-    if (K + 1 < k1 || fuse_last) {
-      const bool tk = (K == timed_K);
-      if (tk) MAFEM_HIP(hipEventRecord(c.ev[10], s));
-      hipLaunchKernelGGL((k_update_diag<S, PERT>), dim3(c.tb * c.tb), dim3(256), 0, s, K, nt, c.tb,
-                         band, piv, c.d_amax);
-      if (tk) {
-        MAFEM_HIP(hipEventRecord(c.ev[11], s));
-        c.step_timed = true;
-      }
-    } else {
-      hipLaunchKernelGGL(k_update<S>, dim3(c.tb * c.tb), dim3(256), 0, s, K, nt, c.tb, band);
+    const FacChain ch{band, piv, K, nt, (K + 1 < k1 || fuse_last) ? 1 : 0, pub};
+    const FacChain none{band, piv, K, 0, 0, pub};
+    const bool tk = (K == timed_K);
+    if (tk) MAFEM_HIP(hipEventRecord(c.ev[10], s));
+    hipLaunchKernelGGL((k_step_la<S, PERT>), dim3(step_blocks(c.tb)), dim3(256), 0, s, c.tb, ch,
+                       none, c.d_amax, c.d_sweep_err);
+    if (tk) {
+      MAFEM_HIP(hipEventRecord(c.ev[11], s));
+      c.step_timed = true;
     }
   }
 }
@@ -1605,7 +1662,7 @@ static void factor_run(Ctx &c, S *band, S *li, S *ui) {
                        (const double *)c.d_bsr, c.d_amax);
   }
   if (c.tw_a == 0) {
-    factor_chain<S, PERT>(c, c.stream, band, c.d_piv, c.nt, 0, c.nt, true, true, true);
+    factor_chain<S, PERT>(c, c.stream, band, c.d_piv, c.d_pub, c.nt, 0, c.nt, true, true, true);
     hipLaunchKernelGGL(k_tile_inv<S>, dim3((unsigned)c.nt), dim3(64), 0, c.stream, c.tb, band, li, ui);
     hipLaunchKernelGGL(k_fold<S>, dim3((unsigned)(c.nt * (2 * c.tb + 1))), dim3(256), 0, c.stream,
                        c.nt, c.tb, band, li, ui);
@@ -1622,17 +1679,24 @@ static void factor_run(Ctx &c, S *band, S *li, S *ui) {
                      c.d_piv, c.d_amax);
   hipLaunchKernelGGL((k_diag_mm<S, PERT>), dim3(1), dim3(256), 0, c.stream, (int64_t)0, c.tb, br, pr,
                      c.d_amax);
+  unsigned long long *pt = c.d_pub, *pb = c.d_pub + 2 * PUBW;
+  hipLaunchKernelGGL(k_fill_sentinel, dim3((4 * PUBW + 255) / 256), dim3(256), 0, c.stream,
+                     (int64_t)(4 * PUBW), c.d_pub);
   const int64_t steps = std::max(a, nb), timed_k = std::min(a, nb) / 2;
+  {
+    const FacChain ct{band, c.d_piv, 0, a > 0 ? rt : 0, 0, pt};
+    const FacChain cb{br, pr, 0, nb > 0 ? rb : 0, 0, pb};
+    hipLaunchKernelGGL(k_panel2<S>, dim3(6 * c.tb), dim3(64), 0, c.stream, c.tb, ct, cb);
+  }
   for (int64_t k = 0; k < steps; ++k) {
     // a chain past its last step drops out (nt = 0); each chain's last step
     // leaves its separator diagonal tile unfactored (fuse = 0)
-    const FacChain ct{band, c.d_piv, k, k < a ? rt : 0, k + 1 < a ? 1 : 0};
-    const FacChain cb{br, pr, k, k < nb ? rb : 0, k + 1 < nb ? 1 : 0};
-    hipLaunchKernelGGL(k_panel2<S>, dim3(6 * c.tb), dim3(64), 0, c.stream, c.tb, ct, cb);
+    const FacChain ct{band, c.d_piv, k, k < a ? rt : 0, k + 1 < a ? 1 : 0, pt};
+    const FacChain cb{br, pr, k, k < nb ? rb : 0, k + 1 < nb ? 1 : 0, pb};
     const bool tk = (k == timed_k);
     if (tk) MAFEM_HIP(hipEventRecord(c.ev[10], c.stream));
-    hipLaunchKernelGGL((k_update_diag2<S, PERT>), dim3(2 * c.tb * c.tb), dim3(256), 0, c.stream,
-                       c.tb, ct, cb, c.d_amax);
+    hipLaunchKernelGGL((k_step_la<S, PERT>), dim3(2 * step_blocks(c.tb)), dim3(256), 0, c.stream,
+                       c.tb, ct, cb, c.d_amax, c.d_sweep_err);
     if (tk) {
       MAFEM_HIP(hipEventRecord(c.ev[11], c.stream));
       c.step_timed = true;
@@ -1640,7 +1704,7 @@ static void factor_run(Ctx &c, S *band, S *li, S *ui) {
   }
   hipLaunchKernelGGL(k_sep_add<S>, dim3((unsigned)(w * w)), dim3(256), 0, c.stream, a, (int)w, c.tb,
                      c.nt, band, (const S *)br);
-  factor_chain<S, PERT>(c, c.stream, band, c.d_piv, rt, a, rt, true, true, false);
+  factor_chain<S, PERT>(c, c.stream, band, c.d_piv, pt, rt, a, rt, true, true, false);
   // inverses of the diagonal tiles; folds: every tile row of BT, B's rows of
   // BR (its separator rows keep their raw L tiles for the solve)
   hipLaunchKernelGGL(k_tile_inv<S>, dim3((unsigned)rt), dim3(64), 0, c.stream, c.tb, band, li, ui);
@@ -1878,7 +1942,7 @@ static void b_factor_t(BatchPlan &B, const std::vector<std::vector<StepMap>> &st
       }
       smp.blk0[sm.ncls] = bp;
       smu.blk0[sm.ncls] = bu;
-      hipLaunchKernelGGL(kb_diag<S>, dim3(nd), dim3(64), 0, B.stream, sm, K, B.d_order, B.d_pd,
+      hipLaunchKernelGGL(kb_diag<S>, dim3(nd), dim3(256), 0, B.stream, sm, K, B.d_order, B.d_pd,
                          f.on, f.fac, f.pert);
       hipLaunchKernelGGL(kb_panel<S>, dim3(bp), dim3(64), 0, B.stream, smp, K, B.d_order, B.d_pd,
                          f.on, f.fac);

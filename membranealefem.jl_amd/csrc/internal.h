@@ -157,6 +157,9 @@ struct Ctx {
   // BT at tile row 0 of d_band / d_linv / d_uinv / d_piv, BR at tile row
   // tw_a + tb.  tw_a = 0: one band factored top to bottom.
   int64_t tw_a = 0, tw_nb = 0;
+  // look-ahead staging of the factorization: per chain (top, bottom) two
+  // buffers of TILE^2 + TILE words, sentinel-filled before each chain runs
+  unsigned long long *d_pub = nullptr;
   double *d_tw_z = nullptr;      // separator right-hand side (original order, nt TILE)
   double *d_tw_br = nullptr;     // right-hand side in reversed order ((tw_nb + tb) TILE)
   double *d_tw_cr = nullptr, *d_tw_dr = nullptr;  // BR forward input / backward input (tw_nb TILE)
