@@ -131,6 +131,115 @@ def batched_flops(patches):
     return tot
 
 
+def bench_batched_solo(args):
+    """Non-lockstep form of the batched workload: the same patches, each
+    advanced through all its steps (warm-up and timed alike) on the
+    single-context direct FP64 path by one of N host threads, largest patches
+    first.  Each context has its own stream and the library releases the
+    interpreter lock inside its calls, so up to N patches' launches overlap on
+    the device and no patch waits for another's Newton iteration.  `value` =
+    Newton iterations of all patches / wall time; each patch's results are
+    those of its one-at-a-time run (contexts share nothing; every sweep split
+    gives bit-identical solutions).  A patch's context is closed when it is
+    done, so at most N bands are held at once."""
+    import queue
+    import threading
+    from mafem.patches import Patch, batched_specs
+
+    specs = batched_specs(args.patches, seed=0xC4, sizes=(args.nmin, args.nmax))
+    t0 = time.perf_counter()
+    patches = []
+    for k, sp in enumerate(specs):
+        patches.append(Patch(sp))
+        if (k + 1) % 64 == 0:
+            print(f"bench: {k + 1}/{len(specs)} patches on the device", file=sys.stderr, flush=True)
+    setup_s = time.perf_counter() - t0
+    dt = 1e-3
+    nsteps = args.warmup + args.steps
+    todo = queue.Queue()
+    for i in sorted(range(len(specs)), key=lambda i: -specs[i].n):
+        todo.put(i)
+    iters = np.zeros(len(specs), dtype=np.int64)
+    lat, failed, errors = [], [], []
+    lock = threading.Lock()
+    done = [0]
+
+    def worker():
+        while True:
+            try:
+                i = todo.get_nowait()
+            except queue.Empty:
+                return
+            pt = patches[i]
+            try:
+                pt.dev.set_solver("direct", precision=64, rtol=1e-13)
+                for k in range(nsteps):
+                    pt.set_pull(pt.spec.speed_at(k))
+                    ts = time.perf_counter()
+                    pt.dev.predict(dt)
+                    rc, eps = pt.dev.time_step((k + 1) * dt, dt)
+                    el = time.perf_counter() - ts
+                    with lock:
+                        lat.append(el)
+                    if rc != 0:
+                        with lock:
+                            failed.append({"patch": int(i), "step": k, "n": specs[i].n,
+                                           "motion": int(specs[i].motion), "kb": specs[i].kb,
+                                           "zv": specs[i].zv, "speed": specs[i].speed})
+                        break
+                    iters[i] += len(eps)
+            except Exception as e:  # reported below
+                with lock:
+                    errors.append((i, e))
+            finally:
+                pt.close()
+            with lock:
+                done[0] += 1
+                if done[0] % 64 == 0:
+                    print(f"bench: {done[0]}/{len(specs)} patches done", file=sys.stderr,
+                          flush=True)
+
+    threads = [threading.Thread(target=worker) for _ in range(args.solo_threads)]
+    t_start = time.perf_counter()
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    t_total = time.perf_counter() - t_start
+    if errors:
+        raise RuntimeError(f"patch {errors[0][0]}: {errors[0][1]}")
+    total = int(iters.sum())
+    numel = np.array([sp.n * sp.n for sp in specs])
+    res = {
+        "metric": METRIC,
+        "value": total / t_total,
+        "unit": "Newton steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_total / nsteps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{len(specs)} independent F_PULL patches, n ~ U{{{args.nmin}.."
+                               f"{args.nmax}}}, kb and zv log-uniform in [0.1, 10], pull ramped "
+                               f"to U[0.2, 3] over 200 steps, motion U{{LAG, ALEVB}}, seed 0xC4, "
+                               f"flat initial state, dt=1e-3; {args.solo_threads} host threads, "
+                               f"one stream each, direct FP64 path per patch, {nsteps} steps "
+                               f"per patch (all timed)",
+                   "patches": len(specs), "elements": int(numel.sum()),
+                   "threads": args.solo_threads, "time_steps": nsteps, "newton_iters": total},
+        "elem_assemblies_per_s": float((iters * numel).sum()) / t_total,
+        "step_latency_ms_p50": 1e3 * float(np.percentile(lat, 50)),
+        "step_latency_ms_p99": 1e3 * float(np.percentile(lat, 99)),
+        "failed_patches": failed,
+        "setup_s": setup_s,
+    }
+    print(json.dumps(res), flush=True)
+
+
 def bench_batched(args):
     """Batched form: many independent patches (sizes, formulations and
     parameters drawn per patch) advanced one time step at a time through one
@@ -263,8 +372,12 @@ def bench_batched(args):
         "gmres_iters": agg["gmres_iters"], "gmres_cycles": agg["gmres_cycles"],
         "hard_failures": agg["hard_failures"], "fp64_patches": fp64_patches,
         "failed_patches": failed,
-        "factorization_tflops": (fact_flops / (agg["factor_us"] * 1e-6 / agg["factor_passes"])
-                                 / 1e12) if agg["factor_passes"] else None,
+        # a pass factors only the members whose preconditioner is refreshed:
+        # the mean algorithmic FLOPs of a member's factorization times the
+        # member factorizations, over the factorization time (FP32 or FP64
+        # band LU per the member's storage)
+        "factorization_tflops": (fact_flops / len(patches) * agg["patch_factors"] /
+                                 (agg["factor_us"] * 1e-6) / 1e12) if agg["factor_us"] else None,
         "plain_fp64": {"newton_per_s": plain_rate, "sample": len(sample),
                        "path": "single-context direct FP64, one patch at a time"},
         "speedup_vs_plain_fp64": value / plain_rate,
@@ -289,6 +402,7 @@ def bench_serving(args):
     specs = serving_specs(args.requests, seed=0xC5)
     reqs = poisson_requests(specs, rate=args.rate, steps=args.req_steps, seed=0xC5)
     srv = Server(precision=args.precision if args.precision else 64, max_members=args.max_members,
+                 solo_min_elements=args.solo_min_n ** 2, max_solo=args.max_solo,
                  refactor_iters=args.refactor_iters if args.refactor_iters is not None else -1)
     out = srv.run(reqs, log=lambda m: print("serve: " + m, file=sys.stderr, flush=True))
     done = [r for r in out if r.finished is not None and r.failed < 0]
@@ -465,6 +579,12 @@ def main():
     ap.add_argument("--requests", type=int, default=48)
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--req-steps", type=int, default=50)
+    ap.add_argument("--solo-threads", type=int, default=0,
+                    help="batched workload: run it non-lockstep on this many host threads "
+                         "(direct FP64 path per patch) instead of one batch")
+    ap.add_argument("--solo-min-n", type=int, default=96,
+                    help="serving: requests of n x n elements or more run solo")
+    ap.add_argument("--max-solo", type=int, default=3, help="serving: concurrent solo requests")
     ap.add_argument("--mixed-steps", type=int, default=1,
                     help="precision workload: steps of the mixed-precision leg")
     ap.add_argument("--max-members", type=int, default=64)
@@ -504,6 +624,9 @@ def main():
         bench_precision(args)
         return
     if args.workload == "batched":
+        if args.solo_threads > 0:
+            bench_batched_solo(args)
+            return
         if args.precision is None:
             args.precision = 32
         bench_batched(args)

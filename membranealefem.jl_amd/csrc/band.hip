@@ -668,6 +668,7 @@ struct FacChain {
   int64_t K, nt;
   int32_t fuse;
   unsigned long long *pub;  // look-ahead staging, 2 PUBW words
+  int64_t k0;               // the chain's first step (its row's L tiles permuted beforehand)
 };
 
 template <class S>
@@ -690,7 +691,9 @@ __global__ __launch_bounds__(64) void k_panel2(int tb, FacChain c0, FacChain c1)
 //                      step K when inside its window, then, once the factors of
 //                      K + 1 are published, L^-1 P X (one wavefront);
 //   tb+1 .. 2 tb     : L panel tiles (I, K+1) likewise, X U^-1;
-//   2 tb+1 .. 3 tb   : P_(K+1) applied to the L tiles of tile row K+1;
+//   2 tb+1 .. 3 tb   : P_(K+1) applied to the L tiles (K+1, J), J < K, and
+//                      P_K to tile (K, K-1) (which the previous launch read
+//                      as an operand, so could not permute);
 //   the rest         : the other (tb - 1)^2 tiles of step K's window.
 // Two staging buffers alternate with the step's parity; block 1 refills the
 // one read in the previous launch with the sentinel.  Every waiting block
@@ -752,6 +755,30 @@ __device__ __forceinline__ void pub_wait(const unsigned long long *__restrict__ 
   __syncthreads();
 }
 
+// P_I applied to the rows of the stored L tile (I, J) by a 256-thread block
+// (all loads before the barrier, then the stores); pv: the 64 pivots of I
+template <class S>
+__device__ __forceinline__ void perm_rows(S *__restrict__ A, const int *pv) {
+  const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+  S v[T / 4];
+#pragma unroll
+  for (int it = 0; it < T / 4; ++it) v[it] = A[(int64_t)pv[4 * it + w] * T + lane];
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < T / 4; ++it) A[(4 * it + w) * T + lane] = v[it];
+}
+
+// P_I on tile (I, I - 1): the one L tile of row I that the look-ahead launch
+// producing the factors of I cannot permute (it is an operand of that launch)
+template <class S>
+__global__ __launch_bounds__(256) void k_perm_tile(int tb, S *__restrict__ band,
+                                                   const int32_t *__restrict__ piv, int64_t I) {
+  __shared__ int pv[T];
+  if (threadIdx.x < T) pv[threadIdx.x] = piv[I * T + threadIdx.x];
+  __syncthreads();
+  perm_rows<S>(tile_ptr(band, tb, I, I - 1), pv);
+}
+
 template <class S, bool PERT>
 __global__ __launch_bounds__(256) void k_step_la(int tb, FacChain c0, FacChain c1,
                                                  unsigned long long *__restrict__ amax,
@@ -768,7 +795,13 @@ __global__ __launch_bounds__(256) void k_step_la(int tb, FacChain c0, FacChain c
   S *band = (S *)ch.band;
   const int64_t K = ch.K, nt = ch.nt;
   if (!ch.fuse) {
-    if (bid < (int64_t)tb * tb) update_body<S>(bid, K, nt, tb, band, lds);
+    if (bid < (int64_t)tb * tb) {
+      update_body<S>(bid, K, nt, tb, band, lds);
+    } else if (bid == (int64_t)tb * tb && K > ch.k0) {  // the deferred P_K of tile (K, K-1)
+      if (threadIdx.x < T) spiv[threadIdx.x] = ch.piv[K * T + threadIdx.x];
+      __syncthreads();
+      perm_rows<S>(tile_ptr(band, tb, K, K - 1), spiv);
+    }
     return;
   }
   unsigned long long *pub_new = ch.pub + ((K + 1) & 1) * PUBW;
@@ -791,17 +824,21 @@ __global__ __launch_bounds__(256) void k_step_la(int tb, FacChain c0, FacChain c
     const int kind = (int)(r / tb), off = (int)(r % tb);
     S(*sX)[T + 1] = reinterpret_cast<S(*)[T + 1]>(lds);
     S(*sD)[T] = reinterpret_cast<S(*)[T]>(lds + T * (T + 1));
-    if (kind == 2) {  // P_(K+1) on the L tile (K+1, J)
-      const int64_t J = K + 1 - tb + off;
+    if (kind == 2) {
+      if (off == tb - 1) {
+        // tile (K+1, K) is step K's L operand of row K+1 in this very launch,
+        // so its P_(K+1) waits for the next one; here: the P_K of (K, K-1)
+        // deferred by the previous launch
+        if (K <= ch.k0) return;
+        if (threadIdx.x < T) spiv[threadIdx.x] = ch.piv[K * T + threadIdx.x];
+        __syncthreads();
+        perm_rows<S>(tile_ptr(band, tb, K, K - 1), spiv);
+        return;
+      }
+      const int64_t J = K + 1 - tb + off;  // P_(K+1) on the L tile (K+1, J), J < K
       if (J < 0) return;
       pub_wait<S>(pub_new, nullptr, spiv, err, &bad);
-      S *A = tile_ptr(band, tb, K + 1, J);
-      S v[T / 4];
-#pragma unroll
-      for (int it = 0; it < T / 4; ++it) v[it] = A[(int64_t)spiv[4 * it + w] * T + lane];
-      __syncthreads();
-#pragma unroll
-      for (int it = 0; it < T / 4; ++it) A[(4 * it + w) * T + lane] = v[it];
+      perm_rows<S>(tile_ptr(band, tb, K + 1, J), spiv);
       return;
     }
     const int64_t IJ = K + 2 + off;
@@ -1638,10 +1675,12 @@ static void factor_chain(Ctx &c, hipStream_t s, S *band, int32_t *piv, unsigned 
                      (int64_t)(2 * PUBW), pub);
   hipLaunchKernelGGL(k_panel<S>, dim3(3 * c.tb), dim3(64), 0, s, k0, nt, c.tb, band, piv);
   const int64_t timed_K = timed ? (k0 + k1) / 2 : -1;
+  int64_t last_lu = k0;  // the last diagonal factored by a look-ahead launch
   for (int64_t K = k0; K < k1; ++K) {
     if (K + 1 >= nt) break;  // no trailing tiles after the last step
-    const FacChain ch{band, piv, K, nt, (K + 1 < k1 || fuse_last) ? 1 : 0, pub};
-    const FacChain none{band, piv, K, 0, 0, pub};
+    const FacChain ch{band, piv, K, nt, (K + 1 < k1 || fuse_last) ? 1 : 0, pub, k0};
+    const FacChain none{band, piv, K, 0, 0, pub, k0};
+    last_lu = ch.fuse ? K + 1 : k0;
     const bool tk = (K == timed_K);
     if (tk) MAFEM_HIP(hipEventRecord(c.ev[10], s));
     hipLaunchKernelGGL((k_step_la<S, PERT>), dim3(step_blocks(c.tb)), dim3(256), 0, s, c.tb, ch,
@@ -1651,6 +1690,8 @@ static void factor_chain(Ctx &c, hipStream_t s, S *band, int32_t *piv, unsigned 
       c.step_timed = true;
     }
   }
+  if (last_lu > k0)  // no later launch applies its P to tile (last_lu, last_lu - 1)
+    hipLaunchKernelGGL(k_perm_tile<S>, dim3(1), dim3(256), 0, s, c.tb, band, piv, last_lu);
 }
 
 template <class S, bool PERT>
@@ -1684,15 +1725,17 @@ static void factor_run(Ctx &c, S *band, S *li, S *ui) {
                      (int64_t)(4 * PUBW), c.d_pub);
   const int64_t steps = std::max(a, nb), timed_k = std::min(a, nb) / 2;
   {
-    const FacChain ct{band, c.d_piv, 0, a > 0 ? rt : 0, 0, pt};
-    const FacChain cb{br, pr, 0, nb > 0 ? rb : 0, 0, pb};
+    const FacChain ct{band, c.d_piv, 0, a > 0 ? rt : 0, 0, pt, 0};
+    const FacChain cb{br, pr, 0, nb > 0 ? rb : 0, 0, pb, 0};
     hipLaunchKernelGGL(k_panel2<S>, dim3(6 * c.tb), dim3(64), 0, c.stream, c.tb, ct, cb);
   }
   for (int64_t k = 0; k < steps; ++k) {
     // a chain past its last step drops out (nt = 0); each chain's last step
     // leaves its separator diagonal tile unfactored (fuse = 0)
-    const FacChain ct{band, c.d_piv, k, k < a ? rt : 0, k + 1 < a ? 1 : 0, pt};
-    const FacChain cb{br, pr, k, k < nb ? rb : 0, k + 1 < nb ? 1 : 0, pb};
+    // (each chain's last step is a plain update, which also applies the
+    // deferred P of its last factored tile row)
+    const FacChain ct{band, c.d_piv, k, k < a ? rt : 0, k + 1 < a ? 1 : 0, pt, 0};
+    const FacChain cb{br, pr, k, k < nb ? rb : 0, k + 1 < nb ? 1 : 0, pb, 0};
     const bool tk = (k == timed_k);
     if (tk) MAFEM_HIP(hipEventRecord(c.ev[10], c.stream));
     hipLaunchKernelGGL((k_step_la<S, PERT>), dim3(2 * step_blocks(c.tb)), dim3(256), 0, c.stream,
