@@ -509,6 +509,25 @@ __device__ __forceinline__ int acc_row(int j) {
 // of the storage precision: numerically zero) is replaced by +-thr (the
 // factorization is then that of a nearby matrix, and the solve finishes with
 // FGMRES) and counted in amax[1].
+// Publication of one panel of a factored diagonal tile for the look-ahead
+// panel solves (PUBW staging words): rows c0..c0+15 of the factors in pivot
+// order (row j = original row prow[j], all 64 columns, final once its panel's
+// pivot steps and U12 are done) and their pivots.  256 threads.
+template <class S>
+__device__ __forceinline__ void pub_stage(unsigned long long *__restrict__ pub, S (*st)[T + 1],
+                                          const int *prow, int c0) {
+  const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+#pragma unroll
+  for (int it = 0; it < 4; ++it) {
+    const int i = c0 + 4 * it + w;
+    __hip_atomic_store(&pub[i * T + lane], pub_bits<S>(st[prow[i]][lane]), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (threadIdx.x < 16)
+    __hip_atomic_store(&pub[T * T + c0 + threadIdx.x], (unsigned long long)prow[c0 + threadIdx.x],
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <class S, bool PERT = false>
 __device__ __forceinline__ void diag_core_mm(int64_t K, S *__restrict__ A, int32_t *__restrict__ piv,
                                              S (*st)[T + 1], int *prow, int *cand,
@@ -556,7 +575,10 @@ __device__ __forceinline__ void diag_core_mm(int64_t K, S *__restrict__ A, int32
       cand[lane] = pos < 0 ? 1 : 0;
     }
     __syncthreads();
-    if (pb == 3) break;
+    if (pb == 3) {
+      if (pub) pub_stage<S>(pub, st, prow, c0);
+      break;
+    }
     if (w == 0 && lane < 48 - c0) {  // U12 = L11^-1 A12, lane = trailing column
       const int c = c0 + 16 + lane;
       S u[16];
@@ -572,6 +594,7 @@ __device__ __forceinline__ void diag_core_mm(int64_t K, S *__restrict__ A, int32
       for (int i = 0; i < 16; ++i) st[prow[c0 + i]][c] = u[i];
     }
     __syncthreads();
+    if (pub) pub_stage<S>(pub, st, prow, c0);  // this panel's pivot rows are final
     {  // A22 -= L21 U12 on the matrix cores, wavefront w: rows 16w .. 16w + 15
       const int row = 16 * w + cc;
       const bool cd = cand[row] != 0;
@@ -604,17 +627,6 @@ __device__ __forceinline__ void diag_core_mm(int64_t K, S *__restrict__ A, int32
   if (w == 0) {
     piv[K * T + pos] = lane;
     if (PERT && npert > 0 && lane == 0) atomicAdd(amax + 1, (unsigned long long)npert);
-  }
-  if (pub) {  // the factors and the pivots for the look-ahead panel first (PUBW words)
-#pragma unroll
-    for (int it = 0; it < T / 4; ++it) {
-      const int i = 4 * it + w;
-      __hip_atomic_store(&pub[i * T + lane], pub_bits<S>(st[prow[i]][lane]), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    }
-    if (w == 0)
-      __hip_atomic_store(&pub[T * T + lane], (unsigned long long)prow[lane], __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
   }
 #pragma unroll
   for (int it = 0; it < T / 4; ++it) {
@@ -750,7 +762,7 @@ __device__ __forceinline__ void pub_wait(const unsigned long long *__restrict__ 
     if (idx < T * T)
       sD[idx / T][idx % T] = pub_val<S>(v[q]);
     else
-      spiv[idx - T * T] = (int)v[q];
+      spiv[idx - T * T] = (int)(v[q] & 63);  // in range even for an expired wait
   }
   __syncthreads();
 }
@@ -777,6 +789,136 @@ __global__ __launch_bounds__(256) void k_perm_tile(int tb, S *__restrict__ band,
   if (threadIdx.x < T) pv[threadIdx.x] = piv[I * T + threadIdx.x];
   __syncthreads();
   perm_rows<S>(tile_ptr(band, tb, I, I - 1), pv);
+}
+
+// Waits (wavefront 0 of a panel block) for stage st of the published factors
+// (pivot rows 16 st .. 16 st + 15 and their pivots) and copies it: rows to
+// sD (pivot order), pivots to spiv.  Bounded: an expired wait counts an error
+// and returns false (the block's result is then invalid).
+template <class S>
+__device__ __forceinline__ bool stage_wait(const unsigned long long *__restrict__ pub, int st,
+                                           S (*sD)[T], int *spiv, int *__restrict__ err) {
+  const int lane = (int)(threadIdx.x & 63);
+  long spins = 0;
+  while (__hip_atomic_load(&pub[T * T + 16 * st + 15], __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT) == SENTINEL) {
+    __builtin_amdgcn_s_sleep(2);
+    if (++spins > SPIN_LIMIT) {
+      if (lane == 0) atomicAdd(err, 1);
+      return false;
+    }
+  }
+  unsigned long long v[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r)
+    v[r] = __hip_atomic_load(&pub[(16 * st + r) * T + lane], __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+  unsigned long long pv = lane < 16 ? __hip_atomic_load(&pub[T * T + 16 * st + lane],
+                                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                    : 0ull;
+  bool ok = true;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    long sp = 0;
+    while (v[r] == SENTINEL) {
+      __builtin_amdgcn_s_sleep(1);
+      v[r] = __hip_atomic_load(&pub[(16 * st + r) * T + lane], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+      if (++sp > SPIN_LIMIT) {
+        ok = false;
+        break;
+      }
+    }
+    sD[16 * st + r][lane] = pub_val<S>(v[r]);
+  }
+  if (lane < 16) {
+    long sp = 0;
+    while (pv == SENTINEL) {
+      __builtin_amdgcn_s_sleep(1);
+      pv = __hip_atomic_load(&pub[T * T + 16 * st + lane], __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+      if (++sp > SPIN_LIMIT) {
+        ok = false;
+        break;
+      }
+    }
+    spiv[16 * st + lane] = (int)(pv & 63);  // in range even for an expired wait
+  }
+  if (!__all(ok)) {
+    if (lane == 0) atomicAdd(err, 1);
+    return false;
+  }
+  __builtin_amdgcn_s_waitcnt(0);  // the LDS rows are read by other lanes next
+  return true;
+}
+
+// The look-ahead panel solve by wavefront 0, stage by stage as the diagonal
+// LU publishes its panels, so all but the last stage overlap the LU.
+// kind 0 (U tile, X <- L^-1 P X; lane = column): X is the updated tile in its
+// original row order; the result row i is stored where its source row
+// spiv[i] was (each source row is read once, before its slot is reused).
+// kind 1 (L tile, X <- X U^-1; lane = row).  Same blocks of 8 and the same
+// order of updates per entry as trsm_wave.
+template <class S>
+__device__ __forceinline__ void staged_solve(int kind, const unsigned long long *__restrict__ pub,
+                                             S (*sD)[T], S (*sX)[T + 1], S *rdiag, int *spiv,
+                                             int *__restrict__ err) {
+  constexpr int B = 8;
+  const int lane = (int)(threadIdx.x & 63);
+  for (int st = 0; st < 4; ++st) {
+    if (!stage_wait<S>(pub, st, sD, spiv, err)) return;
+    if (kind == 0) {
+#pragma unroll 1
+      for (int b = 2 * st; b < 2 * st + 2; ++b) {
+        S xb[B];
+#pragma unroll
+        for (int r = 0; r < B; ++r) xb[r] = sX[spiv[B * b + r]][lane];
+#pragma unroll 1
+        for (int c = 0; c < b; ++c) {
+          S xc[B];
+#pragma unroll
+          for (int q = 0; q < B; ++q) xc[q] = sX[spiv[B * c + q]][lane];
+#pragma unroll
+          for (int q = 0; q < B; ++q)
+#pragma unroll
+            for (int r = 0; r < B; ++r) xb[r] = fma(-sD[B * b + r][B * c + q], xc[q], xb[r]);
+        }
+#pragma unroll
+        for (int q = 0; q < B - 1; ++q)
+#pragma unroll
+          for (int r = q + 1; r < B; ++r) xb[r] = fma(-sD[B * b + r][B * b + q], xb[q], xb[r]);
+#pragma unroll
+        for (int r = 0; r < B; ++r) sX[spiv[B * b + r]][lane] = xb[r];
+      }
+    } else {
+      if (lane < 16) rdiag[16 * st + lane] = S(1) / sD[16 * st + lane][16 * st + lane];
+      __builtin_amdgcn_s_waitcnt(0);
+#pragma unroll 1
+      for (int b = 2 * st; b < 2 * st + 2; ++b) {
+        S xb[B];
+#pragma unroll
+        for (int r = 0; r < B; ++r) xb[r] = sX[lane][B * b + r];
+#pragma unroll 1
+        for (int c = 0; c < b; ++c) {
+          S xc[B];
+#pragma unroll
+          for (int q = 0; q < B; ++q) xc[q] = sX[lane][B * c + q];
+#pragma unroll
+          for (int q = 0; q < B; ++q)
+#pragma unroll
+            for (int r = 0; r < B; ++r) xb[r] = fma(-xc[q], sD[B * c + q][B * b + r], xb[r]);
+        }
+#pragma unroll
+        for (int q = 0; q < B; ++q) {
+          xb[q] *= rdiag[B * b + q];
+#pragma unroll
+          for (int r = q + 1; r < B; ++r) xb[r] = fma(-xb[q], sD[B * b + q][B * b + r], xb[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < B; ++r) sX[lane][B * b + r] = xb[r];
+      }
+    }
+  }
 }
 
 template <class S, bool PERT>
@@ -851,22 +993,15 @@ __global__ __launch_bounds__(256) void k_step_la(int tb, FacChain c0, FacChain c
 #pragma unroll
       for (int it = 0; it < T / 4; ++it) sX[4 * it + w][lane] = A[(4 * it + w) * T + lane];
     }
-    pub_wait<S>(pub_new, sD, spiv, err, &bad);
-    if (kind == 0) {  // P_(K+1): row i of X is row spiv[i] of the updated tile
-      S v[T / 4];
+    if (threadIdx.x < T) spiv[threadIdx.x] = (int)threadIdx.x;  // stays in range on an expired wait
+    __syncthreads();
+    if (w == 0) staged_solve<S>(kind, pub_new, sD, sX, rdiag, spiv, err);
+    __syncthreads();
 #pragma unroll
-      for (int it = 0; it < T / 4; ++it) v[it] = sX[spiv[4 * it + w]][lane];
-      __syncthreads();
-#pragma unroll
-      for (int it = 0; it < T / 4; ++it) sX[4 * it + w][lane] = v[it];
-    } else if (threadIdx.x < T) {
-      rdiag[threadIdx.x] = S(1) / sD[threadIdx.x][threadIdx.x];
+    for (int it = 0; it < T / 4; ++it) {
+      const int i = 4 * it + w;
+      A[i * T + lane] = sX[kind == 0 ? spiv[i] : i][lane];
     }
-    __syncthreads();
-    if (w == 0) trsm_wave<S>(kind, sD, sX, rdiag);
-    __syncthreads();
-#pragma unroll
-    for (int it = 0; it < T / 4; ++it) A[(4 * it + w) * T + lane] = sX[4 * it + w][lane];
     return;
   }
   const int64_t u = r - 3 * (int64_t)tb, tm = tb - 1;
