@@ -487,7 +487,7 @@ def bench_precision(args):
     n = args.n if args.n != 64 else 256
     dt = 1e-3
     total = 500
-    out = {}
+    out, fails = {}, {}
     for mode, nsteps in (("fp64", args.steps), ("mixed", min(args.steps, args.mixed_steps))):
         p, mesh, xms, cps = build_case(n, args.motion, sigma=0.0)
         dev = mafem.DeviceMesh(mesh, p)
@@ -511,9 +511,12 @@ def bench_precision(args):
             dev.predict(dt)
             rc, eps = dev.time_step((k + 1) * dt, dt)
             te = time.perf_counter() - ts
-            if rc != 0:
-                raise RuntimeError(f"{mode}: Newton did not converge at step {k}: {eps}")
             st = dev.stats()
+            if rc != 0:  # the Newton iteration's own failure: reported, the run stops
+                fails[mode] = {"step": k, "eps": [float(e) for e in eps]}
+                print(f"bench: {mode} step {k}: Newton did not converge ({len(eps)} iterations)",
+                      file=sys.stderr, flush=True)
+                break
             print(f"bench: {mode} step {k} (element {el}): {te:.2f} s, {len(eps)} Newton "
                   f"iterations, factorizations {st['n_factor']}, gmres {st['gmres_iters']}, "
                   f"perturbed pivots {st['pivot_perturbations']}", file=sys.stderr, flush=True)
@@ -528,7 +531,7 @@ def bench_precision(args):
     sm, im, tm, stm, _, _ = out["mixed"]
     errs = [max(np.linalg.norm(a[0] - b[0]) / np.linalg.norm(b[0]),
                 np.linalg.norm(a[1] - b[1]) / np.linalg.norm(b[1])) for a, b in zip(sm, sf)]
-    fp64_rate = sum(iF) / sum(tf)
+    fp64_rate = sum(iF) / sum(tf) if tf else None
     mixed_rate = sum(im) / sum(tm) if tm else None
     res = {
         "metric": METRIC,
@@ -537,21 +540,23 @@ def bench_precision(args):
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": 0,
-        "ms_per_step": 1e3 * sum(tf) / args.steps,
+        "ms_per_step": 1e3 * sum(tf) / max(1, len(tf)),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"F_PULL {args.motion} {n}x{n} patch, tether moved across 40% of "
-                               f"the width over 500 steps (elements {els[0]}..{els[-1]} here), "
+                               f"the width over 500 steps (elements {els[0] if els else '-'}.."
+                               f"{els[-1] if els else '-'} here), "
                                f"flat initial state, pull ramped to 2.0 over 200 steps, dt=1e-3; "
                                f"FP64 direct band LU",
                    "nmdf": mesh.nmdf, "numel": mesh.numel},
         "newton_iters_fp64": iF, "newton_iters_mixed": im,
         "per_step_rel_err_mixed_vs_fp64": errs,
         "mixed_newton_per_s": mixed_rate,
-        "mixed_over_fp64": mixed_rate / fp64_rate if mixed_rate else None,
+        "mixed_over_fp64": mixed_rate / fp64_rate if (mixed_rate and fp64_rate) else None,
+        "newton_failures": fails,
         "step_s_fp64": tf, "step_s_mixed": tm,
         "gmres_iters_mixed": [s["gmres_iters"] for s in stm],
         "pivot_perturbations_mixed": [s["pivot_perturbations"] for s in stm],

@@ -852,70 +852,82 @@ __device__ __forceinline__ bool stage_wait(const unsigned long long *__restrict_
   return true;
 }
 
-// The look-ahead panel solve by wavefront 0, stage by stage as the diagonal
-// LU publishes its panels, so all but the last stage overlap the LU.
-// kind 0 (U tile, X <- L^-1 P X; lane = column): X is the updated tile in its
-// original row order; the result row i is stored where its source row
-// spiv[i] was (each source row is read once, before its slot is reused).
-// kind 1 (L tile, X <- X U^-1; lane = row).  Same blocks of 8 and the same
-// order of updates per entry as trsm_wave.
+// The look-ahead panel solve, stage by stage as the diagonal LU publishes its
+// panels, so all but the last stage overlap the LU.  Per stage (16 pivot
+// rows / columns): the contributions of the earlier stages are subtracted on
+// the matrix cores by the four wavefronts (left-looking), then wavefront 0
+// solves the 16 x 16 diagonal block.  kind 0 (U tile, X <- L^-1 P X): X is
+// the updated tile in its original row order and the result row i is stored
+// in the slot of its source row spiv[i] (each source row is read once,
+// before its slot is reused); wavefront w owns columns 16w..16w+15.  kind 1
+// (L tile, X <- X U^-1): wavefront w owns rows 16w..16w+15.  256 threads;
+// bad: an LDS int.
 template <class S>
 __device__ __forceinline__ void staged_solve(int kind, const unsigned long long *__restrict__ pub,
                                              S (*sD)[T], S (*sX)[T + 1], S *rdiag, int *spiv,
-                                             int *__restrict__ err) {
-  constexpr int B = 8;
-  const int lane = (int)(threadIdx.x & 63);
+                                             int *__restrict__ err, int *bad) {
+  using V = typename Mfma<S>::V;
+  const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+  const int g = lane >> 4, cc = lane & 15;
   for (int st = 0; st < 4; ++st) {
-    if (!stage_wait<S>(pub, st, sD, spiv, err)) return;
-    if (kind == 0) {
-#pragma unroll 1
-      for (int b = 2 * st; b < 2 * st + 2; ++b) {
-        S xb[B];
+    const int s0 = 16 * st;
+    if (w == 0) {
+      const bool ok = stage_wait<S>(pub, st, sD, spiv, err);
+      if (lane == 0) *bad = ok ? 0 : 1;
+    }
+    __syncthreads();
+    if (*bad) return;
+    if (st > 0) {
+      V acc;
+      if (kind == 0) {  // rows s0.. (pivot order) -= L[s0.., 0..s0) Y[0..s0), columns 16w..
 #pragma unroll
-        for (int r = 0; r < B; ++r) xb[r] = sX[spiv[B * b + r]][lane];
-#pragma unroll 1
-        for (int c = 0; c < b; ++c) {
-          S xc[B];
+        for (int j = 0; j < 4; ++j) acc[j] = sX[spiv[s0 + acc_row<S>(j)]][16 * w + cc];
+        for (int kb = 0; kb < st; ++kb)
 #pragma unroll
-          for (int q = 0; q < B; ++q) xc[q] = sX[spiv[B * c + q]][lane];
+          for (int kk = 0; kk < 4; ++kk) {
+            const int k = 16 * kb + 4 * g + kk;
+            acc = Mfma<S>::run(-sD[s0 + cc][k], sX[spiv[k]][16 * w + cc], acc);
+          }
 #pragma unroll
-          for (int q = 0; q < B; ++q)
+        for (int j = 0; j < 4; ++j) sX[spiv[s0 + acc_row<S>(j)]][16 * w + cc] = acc[j];
+      } else {  // columns s0.. -= X[:, 0..s0) U[0..s0, s0..), rows 16w..
 #pragma unroll
-            for (int r = 0; r < B; ++r) xb[r] = fma(-sD[B * b + r][B * c + q], xc[q], xb[r]);
-        }
+        for (int j = 0; j < 4; ++j) acc[j] = sX[16 * w + acc_row<S>(j)][s0 + cc];
+        for (int kb = 0; kb < st; ++kb)
 #pragma unroll
-        for (int q = 0; q < B - 1; ++q)
+          for (int kk = 0; kk < 4; ++kk) {
+            const int k = 16 * kb + 4 * g + kk;
+            acc = Mfma<S>::run(-sX[16 * w + cc][k], sD[k][s0 + cc], acc);
+          }
 #pragma unroll
-          for (int r = q + 1; r < B; ++r) xb[r] = fma(-sD[B * b + r][B * b + q], xb[q], xb[r]);
-#pragma unroll
-        for (int r = 0; r < B; ++r) sX[spiv[B * b + r]][lane] = xb[r];
+        for (int j = 0; j < 4; ++j) sX[16 * w + acc_row<S>(j)][s0 + cc] = acc[j];
       }
-    } else {
-      if (lane < 16) rdiag[16 * st + lane] = S(1) / sD[16 * st + lane][16 * st + lane];
-      __builtin_amdgcn_s_waitcnt(0);
-#pragma unroll 1
-      for (int b = 2 * st; b < 2 * st + 2; ++b) {
-        S xb[B];
+      __syncthreads();
+    }
+    if (w == 0) {
+      S x[16];
+      if (kind == 0) {  // unit lower 16 x 16 block, lane = column
 #pragma unroll
-        for (int r = 0; r < B; ++r) xb[r] = sX[lane][B * b + r];
-#pragma unroll 1
-        for (int c = 0; c < b; ++c) {
-          S xc[B];
+        for (int r = 0; r < 16; ++r) x[r] = sX[spiv[s0 + r]][lane];
 #pragma unroll
-          for (int q = 0; q < B; ++q) xc[q] = sX[lane][B * c + q];
+        for (int r = 1; r < 16; ++r)
 #pragma unroll
-          for (int q = 0; q < B; ++q)
+          for (int q = 0; q < r; ++q) x[r] = fma(-sD[s0 + r][s0 + q], x[q], x[r]);
 #pragma unroll
-            for (int r = 0; r < B; ++r) xb[r] = fma(-xc[q], sD[B * c + q][B * b + r], xb[r]);
+        for (int r = 0; r < 16; ++r) sX[spiv[s0 + r]][lane] = x[r];
+      } else {  // upper 16 x 16 block from the right, lane = row
+        if (lane < 16) rdiag[s0 + lane] = S(1) / sD[s0 + lane][s0 + lane];
+        __builtin_amdgcn_s_waitcnt(0);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) x[q] = sX[lane][s0 + q];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+          x[q] *= rdiag[s0 + q];
+#pragma unroll
+          for (int r = q + 1; r < 16; ++r) x[r] = fma(-x[q], sD[s0 + q][s0 + r], x[r]);
         }
 #pragma unroll
-        for (int q = 0; q < B; ++q) {
-          xb[q] *= rdiag[B * b + q];
-#pragma unroll
-          for (int r = q + 1; r < B; ++r) xb[r] = fma(-xb[q], sD[B * b + q][B * b + r], xb[r]);
-        }
-#pragma unroll
-        for (int r = 0; r < B; ++r) sX[lane][B * b + r] = xb[r];
+        for (int q = 0; q < 16; ++q) sX[lane][s0 + q] = x[q];
       }
     }
   }
@@ -995,7 +1007,7 @@ __global__ __launch_bounds__(256) void k_step_la(int tb, FacChain c0, FacChain c
     }
     if (threadIdx.x < T) spiv[threadIdx.x] = (int)threadIdx.x;  // stays in range on an expired wait
     __syncthreads();
-    if (w == 0) staged_solve<S>(kind, pub_new, sD, sX, rdiag, spiv, err);
+    staged_solve<S>(kind, pub_new, sD, sX, rdiag, spiv, err, &bad);
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < T / 4; ++it) {

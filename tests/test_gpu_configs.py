@@ -81,8 +81,10 @@ def test_c3_mixed_precision_solve_256():
     sols = {}
     for prec in (32, 64):
         dev = mafem.DeviceMesh(mesh, p)
+        # both solves to relative residual 1e-12: the solutions' difference is
+        # bounded by cond(K) times the residuals, and cond(K) is ~1e6 or more here
         if prec == 32:
-            dev.set_solver("krylov", precision=32, restart=30, max_iters=2000, rtol=1e-10)
+            dev.set_solver("krylov", precision=32, restart=30, max_iters=2000, rtol=1e-12)
         dev.set_state(xms, cps)
         r, _ = dev.assemble(DT, DT, want_K=False)
         du, relres = dev.solve(-r)
@@ -91,7 +93,7 @@ def test_c3_mixed_precision_solve_256():
               f"factor {st['factor_us'] / 1e3:.0f} ms, solve {st['solve_us'] / 1e3:.0f} ms",
               flush=True)
         assert np.isfinite(du).all()
-        assert relres <= (1e-10 if prec == 32 else 1e-12), (prec, relres)
+        assert relres <= 1e-12, (prec, relres)
         sols[prec] = du
         dev.close()
     assert rel(sols[32], sols[64]) <= 1e-6
