@@ -52,14 +52,15 @@ def pull_speed(k, total=200, final=2.0):
 
 
 # HBM counters per dispatch of the factorization-step kernel (FETCH_SIZE,
-# WRITE_SIZE in KiB, mean over one time step's dispatches of the 64x64 ALEVB
-# bench patch), by kernel; see the roofline below.  None: not measured.
+# WRITE_SIZE in KiB, mean over one time step's two-chain dispatches of the
+# 64x64 ALEVB bench patch), by kernel; see the roofline below.  None: not
+# measured.  profiles/k_step_la_hbm_traffic_pmc.txt holds the runs.
 STEP_COUNTERS_KIB = {
-    "k_step_la": None,
+    "k_step_la": (16013.467625473484, 19804.575757575756),
 }
 # the same counters per dispatch of the area kernel of the bench patch, by motion
 AREA_COUNTERS_KIB = {
-    "ALEVB": None,
+    "ALEVB": (12421.703125, 295202.59375),
 }
 
 
@@ -732,14 +733,14 @@ def main():
     achieved = flops_step / (s_us * 1e-6) / 1e12
     # HBM bytes per k_step_la launch, from two rocprofv3 --pmc passes over
     # `bench.py --steps 1 --warmup 0 --no-cpu-baseline` (FETCH_SIZE, then
-    # WRITE_SIZE, KiB per dispatch averaged over its k_step_la dispatches),
-    # each scaled by the same counter's reading for the 256 MiB copy of
-    # `bench.py --copy-probe` (FETCH_SIZE 131088.375 KiB, WRITE_SIZE 262144 KiB
-    # for 262144 KiB moved).
+    # WRITE_SIZE, KiB per dispatch averaged over its two-chain k_step_la
+    # dispatches), each scaled by the same counter's reading for the 256 MiB
+    # copy of `bench.py --copy-probe` (per 128 MiB copyBuffer dispatch:
+    # FETCH_SIZE 65545 KiB, WRITE_SIZE 131072 KiB for 131072 KiB moved).
     traffic = None
     counters = STEP_COUNTERS_KIB["k_step_la"]
     if counters is not None:
-        traffic = (counters[0] * (262144.0 / 131088.375) + counters[1] * 1.0) * 1024.0
+        traffic = (counters[0] * (131072.0 / 65545.0) + counters[1] * 1.0) * 1024.0
     s_bytes = TILE * TILE * 8
     # algorithmic tiles moved per chain: the window read and written (2 tb^2),
     # step K's L and U panel tiles read (2 tb), the two next-panel tiles outside
@@ -769,7 +770,7 @@ def main():
     a_traffic = None
     if AREA_COUNTERS_KIB.get(args.motion) is not None:
         fk, wk = AREA_COUNTERS_KIB[args.motion]
-        a_traffic = (fk * (262144.0 / 131088.375) + wk * 1.0) * 1024.0
+        a_traffic = (fk * (131072.0 / 65545.0) + wk * 1.0) * 1024.0
     res["area_roofline"] = {"bound": "valu", "achieved": a_tf, "peak": FP64_PEAK_TFLOPS,
                             "unit": "TFLOP/s", "frac": a_tf / FP64_PEAK_TFLOPS,
                             "flops_per_launch": a_flops, "avg_us": area_us,

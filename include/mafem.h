@@ -112,8 +112,10 @@ typedef struct {
   int32_t n_assemble, n_band_solve;  /* launches of the assembly / triangular-solve sweeps */
   int32_t n_factor, gmres_iters;     /* band factorizations / Krylov iterations */
   double step_kernel_us;           /* mean duration of one factorization-step kernel
-                                      (trailing update with the next diagonal-tile LU
-                                      fused; the middle step of each factorization) */
+                                      (trailing update, next diagonal-tile LU and next
+                                      panel; the middle step of each factorization),
+                                      first block start to last block end on the
+                                      device's constant clock */
   double last_relres;              /* |K du + r| / |r| of the last solve */
   int32_t sweep_groups;            /* workgroups of each triangular sweep (autotuned) */
   int32_t pivot_perturbations;     /* diagonal-tile pivots below eps max|K| (numerically

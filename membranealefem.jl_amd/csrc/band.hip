@@ -934,12 +934,9 @@ __device__ __forceinline__ void staged_solve(int kind, const unsigned long long 
 }
 
 template <class S, bool PERT>
-__global__ __launch_bounds__(256) void k_step_la(int tb, FacChain c0, FacChain c1,
-                                                 unsigned long long *__restrict__ amax,
-                                                 int *__restrict__ err) {
-  __shared__ __align__(16) S lds[MM_LDS];
-  __shared__ S rdiag[T];
-  __shared__ int spiv[T];
+__device__ __forceinline__ void step_la_body(int tb, const FacChain &c0, const FacChain &c1,
+                                             unsigned long long *__restrict__ amax,
+                                             int *__restrict__ err, S *lds, S *rdiag, int *spiv) {
   __shared__ int bad;
   const int nbc = step_blocks(tb);
   const bool second = (int)blockIdx.x >= nbc;
@@ -1020,6 +1017,31 @@ __global__ __launch_bounds__(256) void k_step_la(int tb, FacChain c0, FacChain c
   if (u >= tm * tm) return;
   const int64_t I = K + 2 + u / tm, J = K + 2 + u % tm;
   update_body<S>((I - K - 1) * tb + (J - K - 1), K, nt, tb, band, lds);
+}
+
+// span (the timed launch only, else null): the launch's own duration on the
+// device's constant clock, as the earliest block start and latest block end
+template <class S, bool PERT>
+__global__ __launch_bounds__(256) void k_step_la(int tb, FacChain c0, FacChain c1,
+                                                 unsigned long long *__restrict__ amax,
+                                                 int *__restrict__ err,
+                                                 unsigned long long *__restrict__ span) {
+  __shared__ __align__(16) S lds[MM_LDS];
+  __shared__ S rdiag[T];
+  __shared__ int spiv[T];
+  if (span && threadIdx.x == 0) atomicMin(&span[0], (unsigned long long)wall_clock64());
+  step_la_body<S, PERT>(tb, c0, c1, amax, err, lds, rdiag, spiv);
+  if (span) {
+    __syncthreads();
+    if (threadIdx.x == 0) atomicMax(&span[1], (unsigned long long)wall_clock64());
+  }
+}
+
+__global__ void k_span_init(unsigned long long *__restrict__ span) {
+  if (threadIdx.x == 0) {
+    span[0] = ~0ull;
+    span[1] = 0ull;
+  }
 }
 
 // ------------------------------------------------- post-factorization pass
@@ -1811,7 +1833,7 @@ static void fill_t(Ctx &c, S *band) {
 // diagonal LU and the next panel), except after the last step when
 // !fuse_last (its next diagonal tile is not complete yet: plain update).
 // pub: the chain's 2 PUBW staging words.  When `timed`, one launch in the
-// middle is bracketed by ev[10] / ev[11].
+// middle records its duration in c.d_span.
 template <class S, bool PERT>
 static void factor_chain(Ctx &c, hipStream_t s, S *band, int32_t *piv, unsigned long long *pub,
                          int64_t nt, int64_t k0, int64_t k1, bool diag_first, bool fuse_last,
@@ -1829,13 +1851,10 @@ static void factor_chain(Ctx &c, hipStream_t s, S *band, int32_t *piv, unsigned 
     const FacChain none{band, piv, K, 0, 0, pub, k0};
     last_lu = ch.fuse ? K + 1 : k0;
     const bool tk = (K == timed_K);
-    if (tk) MAFEM_HIP(hipEventRecord(c.ev[10], s));
+    if (tk) hipLaunchKernelGGL(k_span_init, dim3(1), dim3(64), 0, s, c.d_span);
     hipLaunchKernelGGL((k_step_la<S, PERT>), dim3(step_blocks(c.tb)), dim3(256), 0, s, c.tb, ch,
-                       none, c.d_amax, c.d_sweep_err);
-    if (tk) {
-      MAFEM_HIP(hipEventRecord(c.ev[11], s));
-      c.step_timed = true;
-    }
+                       none, c.d_amax, c.d_sweep_err, tk ? c.d_span : nullptr);
+    if (tk) c.step_timed = true;
   }
   if (last_lu > k0)  // no later launch applies its P to tile (last_lu, last_lu - 1)
     hipLaunchKernelGGL(k_perm_tile<S>, dim3(1), dim3(256), 0, s, c.tb, band, piv, last_lu);
@@ -1884,13 +1903,10 @@ static void factor_run(Ctx &c, S *band, S *li, S *ui) {
     const FacChain ct{band, c.d_piv, k, k < a ? rt : 0, k + 1 < a ? 1 : 0, pt, 0};
     const FacChain cb{br, pr, k, k < nb ? rb : 0, k + 1 < nb ? 1 : 0, pb, 0};
     const bool tk = (k == timed_k);
-    if (tk) MAFEM_HIP(hipEventRecord(c.ev[10], c.stream));
+    if (tk) hipLaunchKernelGGL(k_span_init, dim3(1), dim3(64), 0, c.stream, c.d_span);
     hipLaunchKernelGGL((k_step_la<S, PERT>), dim3(2 * step_blocks(c.tb)), dim3(256), 0, c.stream,
-                       c.tb, ct, cb, c.d_amax, c.d_sweep_err);
-    if (tk) {
-      MAFEM_HIP(hipEventRecord(c.ev[11], c.stream));
-      c.step_timed = true;
-    }
+                       c.tb, ct, cb, c.d_amax, c.d_sweep_err, tk ? c.d_span : nullptr);
+    if (tk) c.step_timed = true;
   }
   hipLaunchKernelGGL(k_sep_add<S>, dim3((unsigned)(w * w)), dim3(256), 0, c.stream, a, (int)w, c.tb,
                      c.nt, band, (const S *)br);

@@ -115,7 +115,9 @@ struct Ctx {
   int band_prec = 0;
   bool have_factor = false;      // the band holds a factorization usable as preconditioner
   int32_t n_factor = 0;          // running count of factorizations
-  bool step_timed = false;       // ev[10]/ev[11] bracket a timed factorization-step launch not yet read
+  bool step_timed = false;       // d_span holds a timed factorization-step launch not yet read
+  unsigned long long *d_span = nullptr;  // its first block start, last block end (constant clock)
+  double wall_khz = 1e5;         // frequency of that clock
 
   // linear solver options and Krylov workspace
   mafem_solver_opts opts{};
