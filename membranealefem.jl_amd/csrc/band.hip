@@ -1052,35 +1052,38 @@ __global__ void k_span_init(unsigned long long *__restrict__ span) {
 // single batch of tile mat-vecs: y_I = L_II^-1 P_I b_I - Σ L~_IK y_K and
 // x_I = U_II^-1 y_I - Σ U~_IJ x_J.  Every tile is independent: fully parallel.
 
+// Inverses of one factored diagonal tile by one wavefront: the tile staged in
+// LDS, L^-1 = L^-1 I and U^-1 = I U^-1 by the panel's blocked substitutions
+// (trsm_wave), so the factors are read from LDS with wave-uniform addresses.
 template <class S>
 __device__ __forceinline__ void tile_inv_body(int64_t bid, int tb, const S *__restrict__ band,
                                                  S *__restrict__ linv, S *__restrict__ uinv) {
+  __shared__ __align__(16) S sD[T][T];
+  __shared__ S sX[T][T + 1];
+  __shared__ S rdiag[T];
   const int64_t I = bid;
-  const int c = threadIdx.x;
+  const int lane = threadIdx.x;
   const S *D = tile_ptr(band, tb, I, I);
-  S x[T];
-  // L^-1 e_c, L unit lower
 #pragma unroll
-  for (int i = 0; i < T; ++i) {
-    S s = (i == c) ? S(1) : S(0);
-#pragma unroll
-    for (int k = 0; k < i; ++k) s = fma(-D[i * T + k], x[k], s);
-    x[i] = s;
-  }
+  for (int i = 0; i < T; ++i) sD[i][lane] = D[i * T + lane];
+#pragma unroll 8
+  for (int i = 0; i < T; ++i) sX[i][lane] = i == lane ? S(1) : S(0);
+  __syncthreads();
+  trsm_wave<S>(0, sD, sX, rdiag);
+  __syncthreads();
   S *Li = linv + I * (int64_t)T * T;
-#pragma unroll
-  for (int i = 0; i < T; ++i) Li[i * T + c] = x[i];
-  // U^-1 e_c
-#pragma unroll
-  for (int i = T - 1; i >= 0; --i) {
-    S s = (i == c) ? S(1) : S(0);
-#pragma unroll
-    for (int k = i + 1; k < T; ++k) s = fma(-D[i * T + k], x[k], s);
-    x[i] = s / D[i * T + i];
-  }
+#pragma unroll 8
+  for (int i = 0; i < T; ++i) Li[i * T + lane] = sX[i][lane];
+  __syncthreads();
+#pragma unroll 8
+  for (int i = 0; i < T; ++i) sX[i][lane] = i == lane ? S(1) : S(0);
+  rdiag[lane] = S(1) / sD[lane][lane];
+  __syncthreads();
+  trsm_wave<S>(1, sD, sX, rdiag);
+  __syncthreads();
   S *Ui = uinv + I * (int64_t)T * T;
-#pragma unroll
-  for (int i = 0; i < T; ++i) Ui[i * T + c] = x[i];
+#pragma unroll 8
+  for (int i = 0; i < T; ++i) Ui[i * T + lane] = sX[i][lane];
 }
 
 template <class S>
@@ -1977,7 +1980,17 @@ static void solve_tw(Ctx &c, const S *band, const S *li, const S *ui, const doub
   else
     hipLaunchKernelGGL((k_rows_sub<S, false>), dim3((unsigned)w), dim3(256), 0, s, tb, bt, a,
                        (int64_t)0, a, yT, (int64_t)0, c.d_t2, (int64_t)0);
-  hipLaunchKernelGGL(k_fwd_sweep<S>, dim3(1), dim3(SW), 0, s, w, tb, sep, c.d_t2 + a * T);
+  double *ys = c.d_t2 + a * T;  // the separator's rows, swept in place
+  if (mw) {  // over several workgroups too (rows a.. of s1 / s2 are free here)
+    unsigned long long *o = (unsigned long long *)c.d_s1 + a * T;
+    hipLaunchKernelGGL(k_fill_sentinel, dim3(nblk(w * T)), dim3(256), 0, s, w * T, o);
+    hipLaunchKernelGGL(k_fwd_sweep_mw<S>, dim3((unsigned)std::min<int64_t>(W, w)), dim3(SW), 0, s,
+                       w, tb, sep, (const double *)ys, o, c.d_sweep_err);
+    hipLaunchKernelGGL(k_collect, dim3(nblk(w * T)), dim3(256), 0, s, w * T,
+                       (const unsigned long long *)o, ys);
+  } else {
+    hipLaunchKernelGGL(k_fwd_sweep<S>, dim3(1), dim3(SW), 0, s, w, tb, sep, ys);
+  }
   // d = U_II^-1 y (in place per tile row where y is plain)
   if (mw) {
     hipLaunchKernelGGL(k_diag_apply_atomic<S>, dim3((unsigned)a), dim3(256), 0, s, ui, yT, c.d_t2);
@@ -1991,7 +2004,16 @@ static void solve_tw(Ctx &c, const S *band, const S *li, const S *ui, const doub
   }
   hipLaunchKernelGGL(k_diag_apply<S>, dim3((unsigned)w), dim3(256), 0, s, ui + a * tsz,
                      (const int32_t *)nullptr, (const double *)c.d_t2 + a * T, c.d_t2 + a * T);
-  hipLaunchKernelGGL(k_bwd_sweep<S>, dim3(1), dim3(SW), 0, s, w, tb, sep, c.d_t2 + a * T);
+  if (mw) {
+    unsigned long long *o = (unsigned long long *)c.d_s2 + a * T;
+    hipLaunchKernelGGL(k_fill_sentinel, dim3(nblk(w * T)), dim3(256), 0, s, w * T, o);
+    hipLaunchKernelGGL(k_bwd_sweep_mw<S>, dim3((unsigned)std::min<int64_t>(W, w)), dim3(SW), 0, s,
+                       w, tb, sep, (const double *)ys, o, c.d_sweep_err);
+    hipLaunchKernelGGL(k_collect, dim3(nblk(w * T)), dim3(256), 0, s, w * T,
+                       (const unsigned long long *)o, ys);
+  } else {
+    hipLaunchKernelGGL(k_bwd_sweep<S>, dim3(1), dim3(SW), 0, s, w, tb, sep, ys);
+  }
   const int64_t r0 = a - tb > 0 ? a - tb : 0, q0 = nb - tb > 0 ? nb - tb : 0;
   hipLaunchKernelGGL((k_rows_sub<S, false>), dim3((unsigned)(a - r0)), dim3(256), 0, s, tb, bt, r0,
                      a, rt, (const double *)c.d_t2, (int64_t)0, c.d_t2, (int64_t)0);
