@@ -259,8 +259,8 @@ def bench_batched(args):
         patches.append(Patch(sp))
         if (k + 1) % 64 == 0:
             print(f"bench: {k + 1}/{len(specs)} patches on the device", file=sys.stderr, flush=True)
-    if args.refactor_iters is None:
-        args.refactor_iters = 6
+    if args.refactor_iters is None:  # fresh factors each Newton iteration (measured best on C4)
+        args.refactor_iters = 0
     batch = mafem.Batch([pt.dev for pt in patches], precision=args.precision,
                         refactor_iters=args.refactor_iters, use_graphs=not args.no_graphs)
     setup_s = time.perf_counter() - t0

@@ -509,6 +509,19 @@ __device__ __forceinline__ int acc_row(int j) {
 // of the storage precision: numerically zero) is replaced by +-thr (the
 // factorization is then that of a nearby matrix, and the solve finishes with
 // FGMRES) and counted in amax[1].
+// 1 / x from the hardware reciprocal and two Newton steps (within about an
+// ulp; the multipliers of the diagonal LU are scaled by it, as LAPACK's
+// unblocked LU does; 0 gives inf as division would)
+__device__ __forceinline__ double recip_nr(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  y = fma(fma(-x, y, 1.0), y, y);
+  return fma(fma(-x, y, 1.0), y, y);
+}
+__device__ __forceinline__ float recip_nr(float x) {
+  float y = __builtin_amdgcn_rcpf(x);
+  return fmaf(fmaf(-x, y, 1.0f), y, y);
+}
+
 // Publication of one panel of a factored diagonal tile for the look-ahead
 // panel solves (PUBW staging words): rows c0..c0+15 of the factors in pivot
 // order (row j = original row prow[j], all 64 columns, final once its panel's
@@ -563,7 +576,7 @@ __device__ __forceinline__ void diag_core_mm(int64_t K, S *__restrict__ A, int32
           npert += tiny ? 1 : 0;
         }
         if (lane == p) pos = c0 + jj;
-        const S l = (pos < 0) ? r[jj] / pj : S(0);
+        const S l = (pos < 0) ? r[jj] * recip_nr(pj) : S(0);
         if (pos < 0) r[jj] = l;
         if (PERT && tiny && lane == p) r[jj] = pj;
 #pragma unroll
