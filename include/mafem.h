@@ -125,6 +125,10 @@ typedef struct {
   int32_t factor_chains;           /* 2: two-ended band LU (top part downward and bottom
                                       part upward at once, separator last); 1: one chain */
   int32_t separator_rows;          /* 64-row tile rows of the separator (0 with one chain) */
+  int32_t fronts;                  /* MAFEM_ORDER_ND: fronts of the elimination tree (else 0) */
+  int32_t front_steps;             /* MAFEM_ORDER_ND: dependent tile steps of one
+                                      factorization (sum over tree levels of the
+                                      largest front's fully-summed tile rows) */
 } mafem_stats;
 
 /* Builds the device context: uploads the tables, builds the block-sparse
@@ -223,6 +227,23 @@ int mafem_generate_output(mafem_ctx *ctx, double *xout, double *uout);
  * Defaults: DIRECT, FP64, restart 30, 200 iterations, refactor above 20,
  * 3 refinement sweeps, rtol 1e-13. */
 int mafem_set_solver(mafem_ctx *ctx, const mafem_solver_opts *opts);
+
+/* Fill-reducing ordering of the context's factorization (both solver modes).
+ * MAFEM_ORDER_BAND (default): natural node order, band LU (two-ended on long
+ *   patches), the tile rows one dependency chain.
+ * MAFEM_ORDER_ND: nested dissection of the node grid (separators of two node
+ *   lines, recursively), factored as a multifrontal LU: the fronts of each
+ *   level of the elimination tree together, from the leaves up, each a dense
+ *   tile matrix with partial pivoting inside its 64-row diagonal tiles.
+ *   Fewer operations and a far shorter dependency chain than the band (64x64
+ *   ALEVB: 48 dependent tile steps instead of 281), at about twice the
+ *   band's factor storage.
+ * This is the role of UMFPACK's fill-reducing ordering inside `K_gl \ r_gl`
+ * (src/analysis/FiniteElement.jl:38); results agree with the band's to the
+ * solve tolerance, not bit for bit.  Takes effect at the next factorization. */
+#define MAFEM_ORDER_BAND 0
+#define MAFEM_ORDER_ND 1
+int mafem_set_ordering(mafem_ctx *ctx, int32_t ordering);
 int mafem_get_solver(const mafem_ctx *ctx, mafem_solver_opts *opts);
 
 /* Free and total memory of the current device, in bytes (no counterpart in

@@ -21,11 +21,6 @@ std::mutex &mafem::capture_mutex() {
 
 namespace {
 
-struct ArgError {
-  std::string msg;
-  int code;
-};
-
 template <class T>
 T *dalloc(Ctx &c, int64_t n) {
   void *p = nullptr;
@@ -61,6 +56,7 @@ void free_all(Ctx &c) {
                   c.d_tw_cr, c.d_tw_dr, c.d_tw_s1, c.d_tw_s2, c.d_pub, c.d_span};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
+  nd_free(c);
   if (c.h_red) (void)hipHostFree(c.h_red);
   for (auto &e : c.ev)
     if (e) (void)hipEventDestroy(e);
@@ -562,6 +558,17 @@ double solve_refined(Ctx &c, const double *d_b, double *d_x, int max_sweeps, dou
   return rel;
 }
 
+void fill_ordering_stats(const Ctx &c, mafem_stats &st) {
+  const bool nd = c.ordering == 1;
+  st.factor_chains = nd ? 0 : (c.tw_a > 0 ? 2 : 1);
+  st.separator_rows = !nd && c.tw_a > 0 ? c.tb : 0;
+  st.fronts = nd ? nd_fronts(c) : 0;
+  st.front_steps = nd ? nd_steps(c) : 0;
+}
+
+// storage precision of the factorization the context holds
+int factor_prec(const Ctx &c) { return c.ordering == 1 ? nd_prec(c) : c.band_prec; }
+
 // Autotuner of the sweep split: on the first factorization of a context whose
 // sweeps run over several workgroups, time one band solve for each candidate
 // count (2, 4, 8, 16; at most one per four tile rows) and keep the fastest.
@@ -569,6 +576,7 @@ double solve_refined(Ctx &c, const double *d_b, double *d_x, int max_sweeps, dou
 // not depend on the count), so the choice affects time only.  An
 // MAFEM_SWEEP_GROUPS setting disables the tuning.
 void tune_sweeps(Ctx &c, const double *d_b) {
+  if (c.ordering == 1) return;  // the fronts' sweeps have no split to tune
   c.sweep_tuned = true;
   if (c.sweep_groups <= 1 || std::getenv("MAFEM_SWEEP_GROUPS")) return;
   float best = 1e30f;
@@ -626,7 +634,7 @@ double linear_solve(Ctx &c, const double *d_b, double *d_x) {
     return rel2;
   }
   bool fresh = false;
-  if (!c.have_factor || c.band_prec != o.factor_precision) {
+  if (!c.have_factor || factor_prec(c) != o.factor_precision) {
     refactor(c);
     fresh = true;
   }
@@ -935,8 +943,7 @@ int mafem_solve(mafem_ctx *ctx, const double *rhs, double *du, double *relres) {
     c.stats.n_factor = c.n_factor - nf0;
     c.stats.gmres_iters = c.n_gmres - ng0;
     c.stats.sweep_groups = c.sweep_groups;
-    c.stats.factor_chains = c.tw_a > 0 ? 2 : 1;
-    c.stats.separator_rows = c.tw_a > 0 ? c.tb : 0;
+    fill_ordering_stats(c, c.stats);
     return MAFEM_OK;
   });
 }
@@ -997,8 +1004,7 @@ int mafem_time_step(mafem_ctx *ctx, double time, double dt, int32_t max_iter, in
     if (n_step > 0) st.step_kernel_us /= n_step;
     st.sweep_groups = c.sweep_groups;
     st.pivot_perturbations = read_pert(c) - np0;
-    st.factor_chains = c.tw_a > 0 ? 2 : 1;
-    st.separator_rows = c.tw_a > 0 ? c.tb : 0;
+    fill_ordering_stats(c, st);
     c.stats = st;
     if (iters) *iters = it;
     if (!(eps < c.enr)) {
@@ -1107,6 +1113,16 @@ int mafem_set_solver(mafem_ctx *ctx, const mafem_solver_opts *o) {
       return fail(c, "invalid solver options", MAFEM_EINVAL);
     if (o->factor_precision != c.opts.factor_precision) c.have_factor = false;
     c.opts = *o;
+    return MAFEM_OK;
+  });
+}
+
+int mafem_set_ordering(mafem_ctx *ctx, int32_t ordering) {
+  ABI_GUARD(ctx, {
+    if (ordering != MAFEM_ORDER_BAND && ordering != MAFEM_ORDER_ND)
+      return fail(c, "invalid ordering", MAFEM_EINVAL);
+    if (ordering != c.ordering) c.have_factor = false;
+    c.ordering = ordering;
     return MAFEM_OK;
   });
 }

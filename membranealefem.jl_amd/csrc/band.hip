@@ -2127,7 +2127,15 @@ static void ensure_band(Ctx &c, int prec) {
 
 void band_alloc(Ctx &c, int prec) { ensure_band(c, prec); }
 
+// nested-dissection fronts (frontal.inc, below)
+void nd_factor(Ctx &c, int prec, bool perturb);
+void nd_solve(Ctx &c, const double *rhs, double *x);
+
 void band_fill(Ctx &c, int prec) {
+  if (c.ordering == 1) {  // the fronts gather the tangent level by level while factoring
+    c.nd_fill_prec = prec;
+    return;
+  }
   ensure_band(c, prec);
   c.batch_factor_prec = 0;  // the band no longer holds a batch's factorization
   if (prec == 32)
@@ -2138,6 +2146,10 @@ void band_fill(Ctx &c, int prec) {
 
 void band_factor(Ctx &c, bool perturb) {
   ++c.n_factor;
+  if (c.ordering == 1) {
+    nd_factor(c, c.nd_fill_prec, perturb);
+    return;
+  }
   if (c.band_prec == 32) {
     if (perturb) factor_run<float, true>(c, (float *)c.d_band, (float *)c.d_linv, (float *)c.d_uinv);
     else factor_run<float, false>(c, (float *)c.d_band, (float *)c.d_linv, (float *)c.d_uinv);
@@ -2151,6 +2163,10 @@ void band_factor(Ctx &c, bool perturb) {
 
 void band_solve(Ctx &c, const double *d_rhs, double *d_x) {
   ++c.n_band_solve;
+  if (c.ordering == 1) {
+    nd_solve(c, d_rhs, d_x);
+    return;
+  }
   if (c.band_prec == 32)
     solve_t<float>(c, (const float *)c.d_band, (const float *)c.d_linv, (const float *)c.d_uinv,
                    d_rhs, d_x);
@@ -2236,3 +2252,6 @@ void b_band_solve(BatchPlan &B, const uint8_t *mask) {
 }
 
 }  // namespace mafem
+
+// The multifrontal factorization shares the tile bodies above.
+#include "frontal.inc"

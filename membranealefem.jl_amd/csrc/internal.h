@@ -46,6 +46,8 @@ struct EdgeItem {
   double nval;
 };
 
+struct NdDev;  // nested-dissection fronts (band.hip)
+
 struct Ctx {
   std::string err;
   int dev = 0;
@@ -167,6 +169,12 @@ struct Ctx {
   double *d_tw_cr = nullptr, *d_tw_dr = nullptr;  // BR forward input / backward input (tw_nb TILE)
   double *d_tw_s1 = nullptr, *d_tw_s2 = nullptr;  // BR sweep outputs (atomic access only)
   int64_t band_rows() const { return tw_a > 0 ? nt + tb : nt; }
+
+  // fill-reducing ordering of the single-context factorization: 0 = the band
+  // (above), 1 = nested dissection (multifrontal, fronts built on first use)
+  int ordering = 0;
+  NdDev *nd = nullptr;
+  int nd_fill_prec = 64;  // precision band_fill chose for the next front factorization
 
   // staging of mafem_set_prescribed (grown on demand, kept)
   int64_t *d_presc_idx = nullptr;
@@ -338,6 +346,13 @@ void band_fill(Ctx &c, int prec);
 // by +-eps max |K|), used when a factorization without it failed
 void band_factor(Ctx &c, bool perturb = false);
 void band_solve(Ctx &c, const double *d_rhs, double *d_x);
+// nested-dissection fronts (band.hip; band_fill / band_factor / band_solve
+// route to them when Ctx::ordering is 1)
+void nd_free(Ctx &c);
+int nd_prec(const Ctx &c);       // storage precision of the fronts (0: none)
+int nd_fronts(const Ctx &c);     // fronts of the elimination tree (0: not built)
+int nd_steps(const Ctx &c);      // dependent tile steps of one factorization
+int64_t nd_bytes(const Ctx &c);  // bytes of factor storage
 void bsr_spmv(Ctx &c, const double *d_x, double *d_y);
 void vec_axpby(Ctx &c, int64_t n, double a, const double *x, double b, double *y);
 void vec_zero(Ctx &c, int64_t n, double *y);
@@ -365,6 +380,11 @@ struct mafem_ctx {
   } while (0)
 
 namespace mafem {
+// invalid or unsupported input: message and MAFEM_E* code for the C ABI
+struct ArgError {
+  std::string msg;
+  int code;
+};
 struct HipError {
   hipError_t code;
   const char *what, *file;

@@ -57,10 +57,12 @@ class Stats(ctypes.Structure):
         ("last_relres", ctypes.c_double),
         ("sweep_groups", ctypes.c_int32), ("pivot_perturbations", ctypes.c_int32),
         ("factor_chains", ctypes.c_int32), ("separator_rows", ctypes.c_int32),
+        ("fronts", ctypes.c_int32), ("front_steps", ctypes.c_int32),
     ]
 
 
 SOLVER_DIRECT, SOLVER_KRYLOV = 0, 1
+ORDER_BAND, ORDER_ND = 0, 1
 
 
 class SolverOpts(ctypes.Structure):
@@ -113,6 +115,7 @@ SIGNATURES = {
                                        ctypes.c_int32, i32p, f64p]),
     "mafem_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Stats)]),
     "mafem_set_solver": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SolverOpts)]),
+    "mafem_set_ordering": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "mafem_get_solver": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SolverOpts)]),
     "mafem_device_memory": (ctypes.c_int, [ctypes.POINTER(ctypes.c_size_t),
                                            ctypes.POINTER(ctypes.c_size_t)]),

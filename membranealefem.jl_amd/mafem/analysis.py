@@ -172,6 +172,13 @@ class DeviceMesh:
                          int(refine_sweeps), float(rtol))
         self._check(self.lib.mafem_set_solver(self.h, ctypes.byref(o)))
 
+    def set_ordering(self, ordering="band"):
+        """Fill-reducing ordering of the factorization: "band" (natural node
+        order, band LU) or "nd" (nested dissection, multifrontal LU); takes
+        effect at the next factorization."""
+        o = {"band": L.ORDER_BAND, "nd": L.ORDER_ND}[ordering]
+        self._check(self.lib.mafem_set_ordering(self.h, o))
+
     def get_solver(self):
         o = L.SolverOpts()
         self._check(self.lib.mafem_get_solver(self.h, ctypes.byref(o)))
