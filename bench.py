@@ -579,6 +579,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--solver", default="direct", choices=["direct", "krylov"])
+    ap.add_argument("--ordering", default="nd", choices=["band", "nd"],
+                    help="factorization ordering: band LU (natural node order) or nested "
+                         "dissection (multifrontal)")
     ap.add_argument("--precision", type=int, default=None, choices=[64, 32])
     ap.add_argument("--workload", default="single",
                     choices=["single", "batched", "serving", "precision"])
@@ -644,6 +647,7 @@ def main():
     p, mesh, xms, cps = build_case(args.n, args.motion)
     dev = mafem.DeviceMesh(mesh, p)
     dev.set_solver(args.solver, precision=args.precision, rtol=1e-13)
+    dev.set_ordering(args.ordering)
     dev.set_state(xms, cps)
     pres = [(node, mesh.dofs[u]) for (u, node, _v) in mesh.inh_dir_bcs]
     nodes = [a for a, _ in pres]
@@ -696,7 +700,10 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"F_PULL {args.motion} {args.n}x{args.n} patch, pull ramped to 2.0 "
                                f"over 200 steps, dt=1e-3, seeded z-perturbation N(0,1e-6), "
-                               f"FP64 assembly; solver {args.solver}, band-LU factor FP{args.precision}",
+                               f"FP64 assembly; solver {args.solver}, "
+                               + ("band-LU" if args.ordering == "band" else
+                                  "nested-dissection multifrontal LU")
+                               + f" factor FP{args.precision}",
                    "nmdf": mesh.nmdf, "numel": numel, "time_steps": args.steps,
                    "newton_iters": iters},
         "elem_assemblies_per_s": numel * agg["n_assemble"] / (agg["assemble_us"] * 1e-6),
@@ -720,6 +727,10 @@ def main():
     # The two-ended factorization advances both chains (the top part downward,
     # the bottom part upward) in each launch; the separator's tb steps follow
     # as single-chain launches.
+    if args.ordering == "nd":
+        res["roofline"] = nd_roofline(st, agg)
+        finish_single(args, res, agg, mesh, numel, p, xms, cps, dev)
+        return
     _, tb = dev_band_shape(dev)
     chains = st["factor_chains"]
     if chains == 2:
@@ -757,6 +768,23 @@ def main():
                        "factorization_tflops": flops_fact / (agg["factor_us"] * 1e-6 /
                                                              max(1, agg["n_factor"])) / 1e12
                        if agg["factor_us"] > 0 else None}
+    finish_single(args, res, agg, mesh, numel, p, xms, cps, dev)
+
+
+def nd_roofline(st, agg):
+    """Nested-dissection factorization: its algorithmic FLOPs (the library's
+    count over the fronts' tile operations) over the measured factorization
+    time."""
+    per_fact_us = agg["factor_us"] / max(1, agg["n_factor"])
+    achieved = st["front_flops"] / (per_fact_us * 1e-6) / 1e12
+    return {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+            "kernel": "nested-dissection multifrontal factorization (all its launches)",
+            "flops_per_factorization": st["front_flops"], "avg_us": per_fact_us,
+            "fronts": st["fronts"], "dependent_tile_steps": st["front_steps"]}
+
+
+def finish_single(args, res, agg, mesh, numel, p, xms, cps, dev):
     # the area kernel (residual + all tangent columns of every area element,
     # FP64 forward mode on the vector ALUs): algorithmic FLOPs of the counted
     # forward-mode evaluation per element (F_res + F_dir per active direction)

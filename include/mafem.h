@@ -129,6 +129,9 @@ typedef struct {
   int32_t front_steps;             /* MAFEM_ORDER_ND: dependent tile steps of one
                                       factorization (sum over tree levels of the
                                       largest front's fully-summed tile rows) */
+  double front_flops;              /* MAFEM_ORDER_ND: algorithmic FLOPs of one
+                                      factorization (tile products 2*64^3, diagonal
+                                      LUs 2/3*64^3, panel solves 64^3; else 0) */
 } mafem_stats;
 
 /* Builds the device context: uploads the tables, builds the block-sparse

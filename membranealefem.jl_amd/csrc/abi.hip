@@ -564,6 +564,7 @@ void fill_ordering_stats(const Ctx &c, mafem_stats &st) {
   st.separator_rows = !nd && c.tw_a > 0 ? c.tb : 0;
   st.fronts = nd ? nd_fronts(c) : 0;
   st.front_steps = nd ? nd_steps(c) : 0;
+  st.front_flops = nd ? nd_flops(c) : 0.0;
 }
 
 // storage precision of the factorization the context holds

@@ -352,6 +352,7 @@ void nd_free(Ctx &c);
 int nd_prec(const Ctx &c);       // storage precision of the fronts (0: none)
 int nd_fronts(const Ctx &c);     // fronts of the elimination tree (0: not built)
 int nd_steps(const Ctx &c);      // dependent tile steps of one factorization
+double nd_flops(const Ctx &c);   // algorithmic FLOPs of one factorization
 int64_t nd_bytes(const Ctx &c);  // bytes of factor storage
 void bsr_spmv(Ctx &c, const double *d_x, double *d_y);
 void vec_axpby(Ctx &c, int64_t n, double a, const double *x, double b, double *y);
