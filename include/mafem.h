@@ -113,8 +113,9 @@ typedef struct {
   int32_t n_factor, gmres_iters;     /* band factorizations / Krylov iterations */
   double step_kernel_us;           /* mean duration of one factorization-step kernel
                                       (trailing update, next diagonal-tile LU and next
-                                      panel; the middle step of each factorization),
-                                      first block start to last block end on the
+                                      panel; band: the middle step of each factorization,
+                                      nested dissection: every step launch of every
+                                      level), first block start to last block end on the
                                       device's constant clock */
   double last_relres;              /* |K du + r| / |r| of the last solve */
   int32_t sweep_groups;            /* workgroups of each triangular sweep (autotuned) */
@@ -132,6 +133,8 @@ typedef struct {
   double front_flops;              /* MAFEM_ORDER_ND: algorithmic FLOPs of one
                                       factorization (tile products 2*64^3, diagonal
                                       LUs 2/3*64^3, panel solves 64^3; else 0) */
+  double step_flops;               /* algorithmic FLOPs of the launches step_kernel_us
+                                      averages, per launch (same operation counts) */
 } mafem_stats;
 
 /* Builds the device context: uploads the tables, builds the block-sparse

@@ -567,6 +567,13 @@ void fill_ordering_stats(const Ctx &c, mafem_stats &st) {
   st.front_flops = nd ? nd_flops(c) : 0.0;
 }
 
+// algorithmic FLOPs of the band's timed step launch (a middle step: per chain
+// tb^2 tile products, the next diagonal LU and 2 tb panel solves)
+double band_step_flops(const Ctx &c) {
+  const double t3 = (double)TILE * TILE * TILE, tb = c.tb;
+  return (c.tw_a > 0 ? 2.0 : 1.0) * (tb * tb * 2.0 * t3 + 2.0 / 3.0 * t3 + 2.0 * tb * t3);
+}
+
 // storage precision of the factorization the context holds
 int factor_prec(const Ctx &c) { return c.ordering == 1 ? nd_prec(c) : c.band_prec; }
 
@@ -960,6 +967,7 @@ int mafem_time_step(mafem_ctx *ctx, double time, double dt, int32_t max_iter, in
     const int32_t nbs0 = c.n_band_solve, nf0 = c.n_factor, ng0 = c.n_gmres;
     const int32_t np0 = read_pert(c);
     int n_step = 0;
+    double step_fl = 0.0;
     c.step_timed = false;
     MAFEM_HIP(hipEventRecord(c.ev[0], c.stream));
     while (it < max_iter) {
@@ -985,8 +993,18 @@ int mafem_time_step(mafem_ctx *ctx, double time, double dt, int32_t max_iter, in
         if (sp[1] > sp[0]) {
           st.step_kernel_us += 1e3 * (double)(sp[1] - sp[0]) / c.wall_khz;
           ++n_step;
+          step_fl += band_step_flops(c);
         }
         c.step_timed = false;
+      }
+      {
+        double us = 0.0, fl = 0.0;
+        int cnt = 0;
+        if (c.ordering == 1 && nd_read_spans(c, &us, &cnt, &fl)) {
+          st.step_kernel_us += us;
+          n_step += cnt;
+          step_fl += fl;
+        }
       }
       st.last_relres = c.stats.last_relres;
       if (eps_hist) eps_hist[it] = eps;
@@ -1002,7 +1020,10 @@ int mafem_time_step(mafem_ctx *ctx, double time, double dt, int32_t max_iter, in
     st.n_band_solve = c.n_band_solve - nbs0;
     st.n_factor = c.n_factor - nf0;
     st.gmres_iters = c.n_gmres - ng0;
-    if (n_step > 0) st.step_kernel_us /= n_step;
+    if (n_step > 0) {
+      st.step_kernel_us /= n_step;
+      st.step_flops = step_fl / n_step;
+    }
     st.sweep_groups = c.sweep_groups;
     st.pivot_perturbations = read_pert(c) - np0;
     fill_ordering_stats(c, st);

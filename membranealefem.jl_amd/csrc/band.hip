@@ -450,9 +450,56 @@ __device__ __forceinline__ void tile_mm(const S *__restrict__ a, const S *__rest
     for (int j = 0; j < 4; ++j) c[bj][j] = acc[bj][j];
 }
 
+// tile_mm with the A operands read straight into registers (lane l of
+// wavefront w: row 16 w + l % 16, columns 16 (l / 16) .. + 15, contiguous in
+// the row-major tile) and only B staged in LDS: MM1_LDS entries instead of
+// MM_LDS, the same MFMA sequence and operands (bit-identical products).
+constexpr int MM1_LDS = T * (T + 1) + 2 * T;  // B (or a staged tile) + two index arrays
+template <class S>
+__device__ __forceinline__ void tile_mm_ga(const S *__restrict__ a, const S *__restrict__ b,
+                                           S c[4][4], S *lds) {
+  using V = typename Mfma<S>::V;
+  constexpr int P = T + 1;
+  constexpr int W = 16 / sizeof(S);  // entries per 16-byte load
+  S *sb = lds;
+  struct alignas(16) Vec { S v[W]; };
+  const int w = (int)(threadIdx.x >> 6), l = (int)(threadIdx.x & 63);
+  Vec ra[16 / W];
+  const S *ag = a + (16 * w + (l & 15)) * T + 16 * (l >> 4);
+#pragma unroll
+  for (int q = 0; q < 16 / W; ++q) ra[q] = *(const Vec *)(ag + q * W);
+  Vec vb[T * T / W / 256];
+#pragma unroll
+  for (int it = 0; it < T * T / W / 256; ++it)
+    vb[it] = *(const Vec *)(b + (it * 256 + (int)threadIdx.x) * W);
+#pragma unroll
+  for (int it = 0; it < T * T / W / 256; ++it) {
+    const int e = (it * 256 + (int)threadIdx.x) * W;
+    const int r = e / T, col = e % T;
+#pragma unroll
+    for (int q = 0; q < W; ++q) sb[r * P + col + q] = vb[it].v[q];
+  }
+  __syncthreads();
+  const S *br = sb + (16 * (l >> 4)) * P + (l & 15);
+  V acc[4];
+#pragma unroll
+  for (int bj = 0; bj < 4; ++bj) acc[bj] = V{0, 0, 0, 0};
+#pragma unroll
+  for (int kk = 0; kk < 16; ++kk) {
+    const S av = ra[kk / W].v[kk % W];
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj) acc[bj] = Mfma<S>::run(av, br[kk * P + 16 * bj], acc[bj]);
+  }
+#pragma unroll
+  for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) c[bj][j] = acc[bj][j];
+}
+
 // to_lds: the updated tile goes to lds as rows of T + 1 entries (for the
 // diagonal LU that follows in the same block) instead of to the band.
-template <class S>
+// GA: tile_mm_ga (lds of MM1_LDS entries suffices).
+template <class S, bool GA = false>
 __device__ __forceinline__ void update_body(int64_t bid, int64_t K, int64_t nt, int tb, S *__restrict__ band,
                                             S *lds, bool to_lds = false) {
   const int64_t I = K + 1 + bid / tb, J = K + 1 + bid % tb;
@@ -466,7 +513,8 @@ __device__ __forceinline__ void update_body(int64_t bid, int64_t K, int64_t nt, 
 #pragma unroll
     for (int j = 0; j < 4; ++j) old[bj][j] = A[mm_row<S>(j) * T + mm_col<S>(bj)];
   S c[4][4];
-  tile_mm<S>(Lt, Ut, c, lds);
+  if (GA) tile_mm_ga<S>(Lt, Ut, c, lds);
+  else tile_mm<S>(Lt, Ut, c, lds);
   if (to_lds) {
     __syncthreads();  // every wavefront is done with the staged operands
 #pragma unroll
@@ -694,6 +742,9 @@ struct FacChain {
   int32_t fuse;
   unsigned long long *pub;  // look-ahead staging, 2 PUBW words
   int64_t k0;               // the chain's first step (its row's L tiles permuted beforehand)
+  // trailing tiles (I, J) with I, J >= s0 are not updated (a front's Schur
+  // block, which receives all its steps' products at once afterwards)
+  int64_t s0 = INT64_MAX;
 };
 
 template <class S>
@@ -874,7 +925,8 @@ __device__ __forceinline__ bool stage_wait(const unsigned long long *__restrict_
 // in the slot of its source row spiv[i] (each source row is read once,
 // before its slot is reused); wavefront w owns columns 16w..16w+15.  kind 1
 // (L tile, X <- X U^-1): wavefront w owns rows 16w..16w+15.  256 threads;
-// bad: an LDS int.
+// bad: an LDS int.  pub = null: sD and spiv already hold the whole factored
+// tile and its pivots (no waits; bad is not read).
 template <class S>
 __device__ __forceinline__ void staged_solve(int kind, const unsigned long long *__restrict__ pub,
                                              S (*sD)[T], S (*sX)[T + 1], S *rdiag, int *spiv,
@@ -884,12 +936,12 @@ __device__ __forceinline__ void staged_solve(int kind, const unsigned long long 
   const int g = lane >> 4, cc = lane & 15;
   for (int st = 0; st < 4; ++st) {
     const int s0 = 16 * st;
-    if (w == 0) {
+    if (pub && w == 0) {
       const bool ok = stage_wait<S>(pub, st, sD, spiv, err);
       if (lane == 0) *bad = ok ? 0 : 1;
     }
-    __syncthreads();
-    if (*bad) return;
+    __syncthreads();  // also orders the previous stage's diagonal solve before this stage
+    if (pub && *bad) return;
     if (st > 0) {
       V acc;
       if (kind == 0) {  // rows s0.. (pivot order) -= L[s0.., 0..s0) Y[0..s0), columns 16w..
@@ -947,19 +999,15 @@ __device__ __forceinline__ void staged_solve(int kind, const unsigned long long 
 }
 
 template <class S, bool PERT>
-__device__ __forceinline__ void step_la_body(int tb, const FacChain &c0, const FacChain &c1,
-                                             unsigned long long *__restrict__ amax,
-                                             int *__restrict__ err, S *lds, S *rdiag, int *spiv) {
+__device__ __forceinline__ void step_la_chain(int tb, const FacChain &ch, int64_t bid,
+                                              unsigned long long *__restrict__ amax,
+                                              int *__restrict__ err, S *lds, S *rdiag, int *spiv) {
   __shared__ int bad;
-  const int nbc = step_blocks(tb);
-  const bool second = (int)blockIdx.x >= nbc;
-  const FacChain &ch = second ? c1 : c0;
-  if (ch.nt == 0) return;
-  const int64_t bid = (int64_t)blockIdx.x - (second ? nbc : 0);
   S *band = (S *)ch.band;
   const int64_t K = ch.K, nt = ch.nt;
   if (!ch.fuse) {
     if (bid < (int64_t)tb * tb) {
+      if (K + 1 + bid / tb >= ch.s0 && K + 1 + bid % tb >= ch.s0) return;
       update_body<S>(bid, K, nt, tb, band, lds);
     } else if (bid == (int64_t)tb * tb && K > ch.k0) {  // the deferred P_K of tile (K, K-1)
       if (threadIdx.x < T) spiv[threadIdx.x] = ch.piv[K * T + threadIdx.x];
@@ -1029,7 +1077,21 @@ __device__ __forceinline__ void step_la_body(int tb, const FacChain &c0, const F
   const int64_t u = r - 3 * (int64_t)tb, tm = tb - 1;
   if (u >= tm * tm) return;
   const int64_t I = K + 2 + u / tm, J = K + 2 + u % tm;
+  if (I >= ch.s0 && J >= ch.s0) return;
   update_body<S>((I - K - 1) * tb + (J - K - 1), K, nt, tb, band, lds);
+}
+
+// both chains of a two-ended launch: blocks of chain 1 follow chain 0's
+template <class S, bool PERT>
+__device__ __forceinline__ void step_la_body(int tb, const FacChain &c0, const FacChain &c1,
+                                             unsigned long long *__restrict__ amax,
+                                             int *__restrict__ err, S *lds, S *rdiag, int *spiv) {
+  const int nbc = step_blocks(tb);
+  const bool second = (int)blockIdx.x >= nbc;
+  const FacChain &ch = second ? c1 : c0;
+  if (ch.nt == 0) return;
+  step_la_chain<S, PERT>(tb, ch, (int64_t)blockIdx.x - (second ? nbc : 0), amax, err, lds, rdiag,
+                         spiv);
 }
 
 // span (the timed launch only, else null): the launch's own duration on the

@@ -58,7 +58,7 @@ class Stats(ctypes.Structure):
         ("sweep_groups", ctypes.c_int32), ("pivot_perturbations", ctypes.c_int32),
         ("factor_chains", ctypes.c_int32), ("separator_rows", ctypes.c_int32),
         ("fronts", ctypes.c_int32), ("front_steps", ctypes.c_int32),
-        ("front_flops", ctypes.c_double),
+        ("front_flops", ctypes.c_double), ("step_flops", ctypes.c_double),
     ]
 
 
