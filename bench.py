@@ -174,6 +174,7 @@ def bench_batched_solo(args):
             pt = patches[i]
             try:
                 pt.dev.set_solver("direct", precision=64, rtol=1e-13)
+                pt.dev.set_ordering(args.ordering)
                 for k in range(nsteps):
                     pt.set_pull(pt.spec.speed_at(k))
                     ts = time.perf_counter()
@@ -324,6 +325,7 @@ def bench_batched(args):
     for i in sample:
         pt = Patch(specs[i])
         pt.dev.set_solver("direct", precision=64, rtol=1e-13)
+        pt.dev.set_ordering(args.ordering)
         for k in range(args.warmup + args.steps):
             pt.set_pull(pt.spec.speed_at(k))
             ts = time.perf_counter()
@@ -380,7 +382,8 @@ def bench_batched(args):
         "factorization_tflops": (fact_flops / len(patches) * agg["patch_factors"] /
                                  (agg["factor_us"] * 1e-6) / 1e12) if agg["factor_us"] else None,
         "plain_fp64": {"newton_per_s": plain_rate, "sample": len(sample),
-                       "path": "single-context direct FP64, one patch at a time"},
+                       "path": "single-context direct FP64, one patch at a time",
+                       "ordering": args.ordering},
         "speedup_vs_plain_fp64": value / plain_rate,
         "rel_err_vs_fp64_max": max(errs),
         "setup_s": setup_s,
@@ -420,6 +423,7 @@ def bench_serving(args):
     for i in idx:
         pt = Patch(specs[i])
         pt.dev.set_solver("direct", precision=64, rtol=1e-13)
+        pt.dev.set_ordering(args.ordering)
         for k in range(args.plain_steps):
             pt.set_pull(specs[i].speed_at(k))
             ts = time.perf_counter()
@@ -463,7 +467,8 @@ def bench_serving(args):
         "failed_requests": failed,
         "plain_fp64": {"newton_per_s": plain_rate, "sample": int(len(idx)),
                        "steps_each": args.plain_steps,
-                       "path": "single-context direct FP64, one request at a time"},
+                       "path": "single-context direct FP64, one request at a time",
+                       "ordering": args.ordering},
         "speedup_vs_plain_fp64": value / plain_rate,
     }
     print(json.dumps(res), flush=True)
