@@ -235,8 +235,11 @@ int mafem_generate_output(mafem_ctx *ctx, double *xout, double *uout);
 int mafem_set_solver(mafem_ctx *ctx, const mafem_solver_opts *opts);
 
 /* Fill-reducing ordering of the context's factorization (both solver modes).
- * MAFEM_ORDER_BAND (default): natural node order, band LU (two-ended on long
- *   patches), the tile rows one dependency chain.
+ * Default at mafem_create: MAFEM_ORDER_ND from about 16x16 elements up (where
+ * the band would run two-ended), MAFEM_ORDER_BAND below; the environment
+ * variable MAFEM_ORDERING=band|nd at mafem_create overrides it.
+ * MAFEM_ORDER_BAND: natural node order, band LU (two-ended on long patches),
+ *   the tile rows one dependency chain.
  * MAFEM_ORDER_ND: nested dissection of the node grid (separators of two node
  *   lines, recursively), factored as a multifrontal LU: the fronts of each
  *   level of the elimination tree together, from the leaves up, each a dense

@@ -389,6 +389,14 @@ void build(Ctx &c, const mafem_mesh_desc *m, const mafem_params *p) {
     }
   }
   c.band_bytes = c.band_rows() * (2 * c.tb + 1) * (int64_t)TILE * TILE * (int64_t)sizeof(double);
+  // default ordering of the single-context factorization: nested dissection
+  // where the band would be long enough to run two-ended (from about 16x16
+  // elements up), else the band; MAFEM_ORDERING=band|nd overrides
+  c.ordering = c.nt >= 8 * (int64_t)c.tb ? 1 : 0;
+  if (const char *e = std::getenv("MAFEM_ORDERING")) {
+    if (std::strcmp(e, "band") == 0) c.ordering = 0;
+    else if (std::strcmp(e, "nd") == 0) c.ordering = 1;
+  }
 
   // device allocations and uploads
   MAFEM_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));

@@ -247,6 +247,7 @@ def test_multi_workgroup_sweeps_match_single(groups, monkeypatch):
     through sentinel-filled outputs) sum every row in the single-workgroup
     sweep's order: the solutions are bit-identical."""
     print(f"sweep groups {groups}", flush=True)
+    monkeypatch.setenv("MAFEM_ORDERING", "band")  # the band's sweeps
     p, mesh, xms, cps, _ = setup(mafem.ALEVB, 16)
     rhs = np.random.default_rng(3).normal(size=mesh.nmdf)
     sols = []
@@ -269,6 +270,7 @@ def test_multi_workgroup_sweeps_larger_patches(motion, n, groups, monkeypatch):
     refinement) of the Newton right-hand side equals the single-workgroup
     sweep's bit for bit."""
     print(f"sweep groups {groups} motion={motion.name} n={n}", flush=True)
+    monkeypatch.setenv("MAFEM_ORDERING", "band")  # the band's sweeps
     p = mafem.Params(motion=motion, num1el=n, num2el=n, length=1.0, enr=1e-10, output=False)
     mesh = mafem.make_mesh(p, pull_speed=0.3)
     xms, cps = mafem.prepare_input(p, mesh, pull_speed=0.3)
@@ -286,12 +288,13 @@ def test_multi_workgroup_sweeps_larger_patches(motion, n, groups, monkeypatch):
     assert np.array_equal(sols[1], sols[0])
 
 
-def test_direct_solve_on_a_band_beyond_one_dispatch():
+def test_direct_solve_on_a_band_beyond_one_dispatch(monkeypatch):
     """A 256x256 ALEVB patch: its FP64 band holds 8321 x 131 x 64^2 = 4.5e9
     entries (36 GB), more than a dispatch's 32-bit work-item count.  The band
     is zeroed by a grid-stride kernel over a capped grid, and the direct solve
     of the Newton right-hand side is finite and meets the refinement target."""
     print("256x256 ALEVB direct solve", flush=True)
+    monkeypatch.setenv("MAFEM_ORDERING", "band")
     p = mafem.Params(motion=mafem.ALEVB, num1el=256, num2el=256, length=1.0, enr=1e-10,
                      output=False)
     mesh = mafem.make_mesh(p, pull_speed=0.3)
@@ -313,6 +316,7 @@ def test_sweep_autotuner_changes_time_only(monkeypatch):
     candidate gives bit-identical results, so a time step with the tuned
     split equals one with a fixed split."""
     print("sweep autotuner", flush=True)
+    monkeypatch.setenv("MAFEM_ORDERING", "band")  # the band's sweep split
     p = mafem.Params(motion=mafem.LAG, num1el=64, num2el=64, length=1.0, enr=1e-10, output=False)
     mesh = mafem.make_mesh(p, pull_speed=0.3)
     xms0, cps0 = mafem.prepare_input(p, mesh, pull_speed=0.3)

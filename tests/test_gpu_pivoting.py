@@ -19,13 +19,15 @@ from mafem.patches import Patch, serving_specs
 pytestmark = pytest.mark.gpu
 
 
-def test_zero_pivot_tangent_converges_with_static_pivoting():
-    print("static pivoting: 96x96 EUL request", flush=True)
+@pytest.mark.parametrize("ordering", ["band", "nd"])
+def test_zero_pivot_tangent_converges_with_static_pivoting(ordering):
+    print(f"static pivoting: 96x96 EUL request, {ordering} ordering", flush=True)
     sp = serving_specs(24)[12]
     assert int(sp.motion) == int(mafem.EUL) and sp.n == 96
     pt = Patch(sp)
     try:
         pt.dev.set_solver("direct", precision=64)
+        pt.dev.set_ordering(ordering)
         pt.set_pull(sp.speed_at(0))
         pt.dev.predict(1e-3)
         rc, eps = pt.dev.time_step(1e-3, 1e-3)
@@ -35,7 +37,8 @@ def test_zero_pivot_tangent_converges_with_static_pivoting():
     print(f"  eps {eps} perturbations {st['pivot_perturbations']}", flush=True)
     assert rc == 0 and len(eps) == 3
     assert eps[-1] < 1e-10 and abs(eps[1] - 3.76e-5) < 1e-7
-    assert st["pivot_perturbations"] > 0
+    if ordering == "band":  # its 64-row diagonal tiles meet numerically zero pivots
+        assert st["pivot_perturbations"] > 0
 
 
 @pytest.mark.parametrize("precision", [32, 64])

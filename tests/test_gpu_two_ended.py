@@ -35,7 +35,9 @@ def _device(mesh, p, two_ended):
     old = os.environ.get("MAFEM_TWO_ENDED")
     os.environ["MAFEM_TWO_ENDED"] = "1" if two_ended else "0"
     try:
-        return DeviceMesh(mesh, p)
+        dev = DeviceMesh(mesh, p)
+        dev.set_ordering("band")  # these tests are about the band's two layouts
+        return dev
     finally:
         if old is None:
             del os.environ["MAFEM_TWO_ENDED"]
