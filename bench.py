@@ -52,11 +52,13 @@ def pull_speed(k, total=200, final=2.0):
 
 
 # HBM counters per dispatch of the factorization-step kernel (FETCH_SIZE,
-# WRITE_SIZE in KiB, mean over one time step's two-chain dispatches of the
-# 64x64 ALEVB bench patch), by kernel; see the roofline below.  None: not
-# measured.  profiles/k_step_la_hbm_traffic_pmc.txt holds the runs.
+# WRITE_SIZE in KiB, mean over one time step's dispatches of the 64x64 ALEVB
+# bench patch), by kernel; see the rooflines below.  None: not measured.
+# profiles/k_step_la_hbm_traffic_pmc.txt and profiles/nd_hbm_traffic_pmc.txt
+# hold the runs.
 STEP_COUNTERS_KIB = {
     "k_step_la": (16013.467625473484, 19804.575757575756),
+    "knd_step_la": (5688.40875, 5070.185),
 }
 # the same counters per dispatch of the area kernel of the bench patch, by motion
 AREA_COUNTERS_KIB = {
@@ -794,11 +796,19 @@ def nd_roofline(st_list, agg):
     w = [s for s in st_list if s["step_kernel_us"] > 0]
     us = float(np.mean([s["step_kernel_us"] for s in w])) if w else float("nan")
     fl = float(np.mean([s["step_flops"] for s in w])) if w else float("nan")
+    by = float(np.mean([s["step_bytes"] for s in w])) if w else float("nan")
     achieved = fl / (us * 1e-6) / 1e12
+    # HBM bytes per launch from FETCH_SIZE / WRITE_SIZE per dispatch (mean
+    # over one time step's knd_step_la dispatches), scaled as for k_step_la
+    traffic = None
+    counters = STEP_COUNTERS_KIB["knd_step_la"]
+    if counters is not None:
+        traffic = (counters[0] * (131072.0 / 65545.0) + counters[1] * 1.0) * 1024.0
     st = st_list[-1]
     per_fact_us = agg["factor_us"] / max(1, agg["n_factor"])
     return {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+            "traffic_unit": "bytes per launch", "algorithmic_bytes": by,
             "kernel": ("knd_step_la (one tile step of every front of a nested-dissection "
                        "level: trailing update on the matrix cores, the next 64x64 diagonal "
                        "LU and the next panel in the same launch)"),

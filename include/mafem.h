@@ -135,6 +135,8 @@ typedef struct {
                                       LUs 2/3*64^3, panel solves 64^3; else 0) */
   double step_flops;               /* algorithmic FLOPs of the launches step_kernel_us
                                       averages, per launch (same operation counts) */
+  double step_bytes;               /* MAFEM_ORDER_ND: their algorithmic bytes per launch
+                                      (tiles read and written once; else 0) */
 } mafem_stats;
 
 /* Builds the device context: uploads the tables, builds the block-sparse

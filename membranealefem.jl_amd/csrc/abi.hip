@@ -975,7 +975,7 @@ int mafem_time_step(mafem_ctx *ctx, double time, double dt, int32_t max_iter, in
     const int32_t nbs0 = c.n_band_solve, nf0 = c.n_factor, ng0 = c.n_gmres;
     const int32_t np0 = read_pert(c);
     int n_step = 0;
-    double step_fl = 0.0;
+    double step_fl = 0.0, step_by = 0.0;
     c.step_timed = false;
     MAFEM_HIP(hipEventRecord(c.ev[0], c.stream));
     while (it < max_iter) {
@@ -1006,12 +1006,13 @@ int mafem_time_step(mafem_ctx *ctx, double time, double dt, int32_t max_iter, in
         c.step_timed = false;
       }
       {
-        double us = 0.0, fl = 0.0;
+        double us = 0.0, fl = 0.0, by = 0.0;
         int cnt = 0;
-        if (c.ordering == 1 && nd_read_spans(c, &us, &cnt, &fl)) {
+        if (c.ordering == 1 && nd_read_spans(c, &us, &cnt, &fl, &by)) {
           st.step_kernel_us += us;
           n_step += cnt;
           step_fl += fl;
+          step_by += by;
         }
       }
       st.last_relres = c.stats.last_relres;
@@ -1031,6 +1032,7 @@ int mafem_time_step(mafem_ctx *ctx, double time, double dt, int32_t max_iter, in
     if (n_step > 0) {
       st.step_kernel_us /= n_step;
       st.step_flops = step_fl / n_step;
+      st.step_bytes = step_by / n_step;
     }
     st.sweep_groups = c.sweep_groups;
     st.pivot_perturbations = read_pert(c) - np0;

@@ -353,8 +353,8 @@ int nd_prec(const Ctx &c);       // storage precision of the fronts (0: none)
 int nd_fronts(const Ctx &c);     // fronts of the elimination tree (0: not built)
 int nd_steps(const Ctx &c);      // dependent tile steps of one factorization
 double nd_flops(const Ctx &c);   // algorithmic FLOPs of one factorization
-// durations (sum, us) / count / FLOPs of the last factorization's step launches
-bool nd_read_spans(Ctx &c, double *us, int *count, double *flops);
+// durations (sum, us) / count / FLOPs / bytes of the last factorization's step launches
+bool nd_read_spans(Ctx &c, double *us, int *count, double *flops, double *bytes);
 int64_t nd_bytes(const Ctx &c);  // bytes of factor storage
 void bsr_spmv(Ctx &c, const double *d_x, double *d_y);
 void vec_axpby(Ctx &c, int64_t n, double a, const double *x, double b, double *y);
