@@ -602,9 +602,12 @@ def main():
     ap.add_argument("--solo-threads", type=int, default=0,
                     help="batched workload: run it non-lockstep on this many host threads "
                          "(direct FP64 path per patch) instead of one batch")
-    ap.add_argument("--solo-min-n", type=int, default=96,
-                    help="serving: requests of n x n elements or more run solo")
-    ap.add_argument("--max-solo", type=int, default=3, help="serving: concurrent solo requests")
+    ap.add_argument("--solo-min-n", type=int, default=32,
+                    help="serving: requests of n x n elements or more run solo (each on its own "
+                         "stream with the nested-dissection direct path; the default sends every "
+                         "request of the stream, n >= 32, solo: 54 vs 29 Newton steps/s with "
+                         "n >= 96 solo and the rest batched)")
+    ap.add_argument("--max-solo", type=int, default=16, help="serving: concurrent solo requests")
     ap.add_argument("--mixed-steps", type=int, default=1,
                     help="precision workload: steps of the mixed-precision leg")
     ap.add_argument("--max-members", type=int, default=64)

@@ -91,8 +91,10 @@ class Server:
         self.solo_count = 0
 
     def _bytes(self, r):
-        # FP64 band storage in the estimate: a member may fall back to it
-        return int(1.15 * device_bytes(r.spec, precision=64))
+        # FP64 band storage in the estimate (a member may fall back to it);
+        # requests large enough to run solo factor with nested dissection
+        solo = r.spec.n * r.spec.n >= self.solo_min_elements
+        return int(1.15 * device_bytes(r.spec, precision=64, ordering="nd" if solo else "band"))
 
     def run(self, requests, time_limit=None, log=None):
         """Serve `requests` (sorted by arrival); returns them with results."""
@@ -121,7 +123,7 @@ class Server:
         def solo(r, pt):
             try:
                 pt.dev.set_solver("direct", precision=64)
-                pt.dev.set_ordering("band")  # the admission budget counts band storage
+                pt.dev.set_ordering("nd")
                 for k in range(r.steps):
                     pt.set_pull(pt.spec.speed_at(k))
                     ts = time.perf_counter()

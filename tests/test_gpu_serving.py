@@ -32,7 +32,7 @@ def requests():
 def plain(spec, steps):
     pt = Patch(spec)
     pt.dev.set_solver("direct", precision=64, rtol=1e-13)
-    pt.dev.set_ordering("band")  # as the server's solo requests
+    pt.dev.set_ordering("nd")  # as the server's solo requests
     its = []
     for k in range(steps):
         pt.set_pull(spec.speed_at(k))
