@@ -62,7 +62,7 @@ STEP_COUNTERS_KIB = {
 }
 # the same counters per dispatch of the area kernel of the bench patch, by motion
 AREA_COUNTERS_KIB = {
-    "ALEVB": (12421.703125, 295202.59375),
+    "ALEVB": (5809.15625, 168437.71875),
 }
 
 
@@ -816,7 +816,6 @@ def nd_roofline(st_list, agg):
                        "level: trailing update on the matrix cores, the next 64x64 diagonal "
                        "LU and the next panel in the same launch)"),
             "flops_per_launch": fl, "avg_us": us,
-            "launches_per_factorization": st["front_steps"],
             "factorization_tflops": st["front_flops"] / (per_fact_us * 1e-6) / 1e12,
             "flops_per_factorization": st["front_flops"], "fronts": st["fronts"],
             "dependent_tile_steps": st["front_steps"]}

@@ -35,6 +35,28 @@ __device__ __forceinline__ D dsqrt(D a) {
   return D{s, a.t / (2.0 * s)};
 }
 
+// Real numbers with the dual numbers' interface: the derivative part reads as
+// zero and is never computed (the area kernel's primal pass).
+struct R {
+  double v;
+  static constexpr double t = 0.0;
+};
+__device__ __forceinline__ R operator+(R a, R b) { return R{a.v + b.v}; }
+__device__ __forceinline__ R operator-(R a, R b) { return R{a.v - b.v}; }
+__device__ __forceinline__ R operator-(R a) { return R{-a.v}; }
+__device__ __forceinline__ R operator*(R a, R b) { return R{a.v * b.v}; }
+__device__ __forceinline__ R operator*(double s, R a) { return R{s * a.v}; }
+__device__ __forceinline__ R operator*(R a, double s) { return R{s * a.v}; }
+__device__ __forceinline__ R recip(R a) { return R{1.0 / a.v}; }
+__device__ __forceinline__ R dsqrt(R a) { return R{sqrt(a.v)}; }
+
+template <class N>
+__device__ __forceinline__ N mkN(double v, double t);
+template <>
+__device__ __forceinline__ D mkN<D>(double v, double t) { return D{v, t}; }
+template <>
+__device__ __forceinline__ R mkN<R>(double v, double) { return R{v}; }
+
 // DOF layout of mesh.dofs for each motion (get_dofs, src/input/Bc.jl:414-431)
 template <int MO>
 struct Lay {
@@ -56,15 +78,43 @@ struct Lay {
 };
 
 // Geometry of one Gauss point with the lane's seed (GeoDynStress.jl:111-130).
+template <class N = D>
 struct Geo {
-  D a1[3], a2[3], n[3], au1[3], au2[3], J;
-  D g00, g01, g11;  // a^{αβ}
-  D idet;           // 1 / det a_{αβ}
+  N a1[3], a2[3], n[3], au1[3], au2[3], J;
+  N g00, g01, g11;  // a^{αβ}
+  N idet;           // 1 / det a_{αβ}
 };
 
+// Passes of the area kernel.  PLAIN: dual numbers throughout.  PRIMAL: one
+// lane per (element, Gauss point) evaluates the point's values in real
+// arithmetic and keeps them in LDS (slots below).  TANGENT: the lanes of the
+// tangent columns take those values as the value parts of their dual numbers,
+// so only the derivative parts are computed per column (sums over the nine
+// nodes, quotients and the square root of the metric are not repeated).
+enum { PLAIN = 0, PRIMAL = 1, TANGENT = 2 };
+enum {
+  S_A1 = 0, S_A2 = 3, S_N = 6, S_AU1 = 9, S_AU2 = 12, S_J = 15, S_IJ = 16, S_G00 = 17,
+  S_G01 = 18, S_G11 = 19, S_IDET = 20, S_X11 = 21, S_X22 = 24, S_X12 = 27, S_GA0 = 30,
+  S_GA1 = 33, S_BB = 36, S_H = 39, S_KC = 40, S_B = 41, S_V = 45, S_DV0 = 48, S_DV1 = 51,
+  S_LAM = 54, S_SG = 55, S_MV = 58, S_VM = 61, S_DVM0 = 64, S_DVM1 = 67, S_PM = 70,
+  S_SGM = 71, N_SLOTS = 74
+};
+
+template <int PS, class N>
+__device__ __forceinline__ void fix(N &x, double *pr, int k) {
+  if constexpr (PS == PRIMAL) pr[k] = x.v;
+  else if constexpr (PS == TANGENT) x.v = pr[k];
+}
+template <int PS, class N, int M>
+__device__ __forceinline__ void fixa(N (&x)[M], double *pr, int k) {
+#pragma unroll
+  for (int i = 0; i < M; ++i) fix<PS>(x[i], pr, k + i);
+}
+
+template <class N = D, int PS = PLAIN>
 __device__ __forceinline__ void geometry(const double *X /*[3][9]*/, const double *bN1,
                                          const double *bN2, int a0, int i0, double sX,
-                                         Geo &g) {
+                                         Geo<N> &g, double *pr = nullptr) {
   const double t1 = sX * bN1[a0], t2 = sX * bN2[a0];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
@@ -74,34 +124,298 @@ __device__ __forceinline__ void geometry(const double *X /*[3][9]*/, const doubl
       s1 = fma(X[i * 9 + a], bN1[a], s1);
       s2 = fma(X[i * 9 + a], bN2[a], s2);
     }
-    g.a1[i] = mk(s1, i == i0 ? t1 : 0.0);
-    g.a2[i] = mk(s2, i == i0 ? t2 : 0.0);
+    g.a1[i] = mkN<N>(s1, i == i0 ? t1 : 0.0);
+    g.a2[i] = mkN<N>(s2, i == i0 ? t2 : 0.0);
   }
-  D a11 = g.a1[0] * g.a1[0] + g.a1[1] * g.a1[1] + g.a1[2] * g.a1[2];
-  D a12 = g.a1[0] * g.a2[0] + g.a1[1] * g.a2[1] + g.a1[2] * g.a2[2];
-  D a22 = g.a2[0] * g.a2[0] + g.a2[1] * g.a2[1] + g.a2[2] * g.a2[2];
-  D det = a11 * a22 - a12 * a12;
-  g.idet = recip(det);
+  fixa<PS>(g.a1, pr, S_A1);
+  fixa<PS>(g.a2, pr, S_A2);
+  N a11 = g.a1[0] * g.a1[0] + g.a1[1] * g.a1[1] + g.a1[2] * g.a1[2];
+  N a12 = g.a1[0] * g.a2[0] + g.a1[1] * g.a2[1] + g.a1[2] * g.a2[2];
+  N a22 = g.a2[0] * g.a2[0] + g.a2[1] * g.a2[1] + g.a2[2] * g.a2[2];
+  N det = a11 * a22 - a12 * a12;
+  N iJ;
+  if constexpr (PS == TANGENT) {
+    // d(1/det) = -d det / det^2, d sqrt(det) = d det / (2 sqrt(det)), d(1/J) = -dJ / J^2
+    const double idv = pr[S_IDET], jv = pr[S_J], ijv = pr[S_IJ];
+    g.idet = N{idv, -det.t * idv * idv};
+    g.J = N{jv, det.t * (0.5 * ijv)};
+    iJ = N{ijv, -g.J.t * ijv * ijv};
+  } else {
+    g.idet = recip(det);
+    g.J = dsqrt(det);
+    iJ = recip(g.J);
+    fix<PS>(g.idet, pr, S_IDET);
+    fix<PS>(g.J, pr, S_J);
+    fix<PS>(iJ, pr, S_IJ);
+  }
   g.g00 = a22 * g.idet;
   g.g01 = -(a12 * g.idet);
   g.g11 = a11 * g.idet;
+  fix<PS>(g.g00, pr, S_G00);
+  fix<PS>(g.g01, pr, S_G01);
+  fix<PS>(g.g11, pr, S_G11);
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     g.au1[i] = g.a1[i] * g.g00 + g.a2[i] * g.g01;
     g.au2[i] = g.a1[i] * g.g01 + g.a2[i] * g.g11;
   }
-  g.J = dsqrt(det);
-  D iJ = recip(g.J);
+  fixa<PS>(g.au1, pr, S_AU1);
+  fixa<PS>(g.au2, pr, S_AU2);
   g.n[0] = (g.a1[1] * g.a2[2] - g.a1[2] * g.a2[1]) * iJ;
   g.n[1] = (g.a1[2] * g.a2[0] - g.a1[0] * g.a2[2]) * iJ;
   g.n[2] = (g.a1[0] * g.a2[1] - g.a1[1] * g.a2[0]) * iJ;
+  fixa<PS>(g.n, pr, S_N);
 }
 
-// LIST = false: elements 0..numel-1, outputs K_el and r_el per element.
-// LIST = true: the numel elements elist[0..numel-1], output r_el only, at the
-// element's position in the list.  P lanes share each tangent column (lane
-// part p takes the rows of nodes p NPP .. p NPP + NPP - 1, NPP = ceil(9 / P));
-// the part holding the column's own row writes its r_el entry.
+// One Gauss point of the area element (GeoDynStress.jl:111-159,
+// FiniteElement.jl:284-313).  PRIMAL ends after the stresses, with the point's
+// values in pr; PLAIN and TANGENT add the lane's rows (nodes a_lo .. a_lo +
+// NPP - 1) of the tangent column to acc and its own row's residual to racc.
+template <int MO, class N, int PS, int P, int NR>
+__device__ __forceinline__ void gp_eval(const double *X, const double *C, const double *rec,
+                                        int a0, int d0, int i0, int a_lo, int col, double sX0,
+                                        double sC0, const PhysParams &ph, double azv, double *pr,
+                                        double (&acc)[NR], double &racc) {
+  using L = Lay<MO>;
+  constexpr int NDF = L::NDF, NPP = (9 + P - 1) / P;
+  const double w = rec[0];
+  const double *bN = rec + 1, *bN1 = rec + 10, *bN2 = rec + 19;
+  const double *b11 = rec + 28, *b22 = rec + 37, *b12 = rec + 46;
+
+  Geo<N> g;
+  geometry<N, PS>(X, bN1, bN2, a0, i0, sX0, g, pr);
+  // second derivatives of position: x_{,11}, x_{,22}, x_{,12}
+  N x11[3], x22[3], x12[3];
+  {
+    const double t11 = sX0 * b11[a0], t22 = sX0 * b22[a0], t12 = sX0 * b12[a0];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double s11 = 0, s22 = 0, s12 = 0;
+#pragma unroll
+      for (int a = 0; a < 9; ++a) {
+        s11 = fma(X[i * 9 + a], b11[a], s11);
+        s22 = fma(X[i * 9 + a], b22[a], s22);
+        s12 = fma(X[i * 9 + a], b12[a], s12);
+      }
+      x11[i] = mkN<N>(s11, i == i0 ? t11 : 0.0);
+      x22[i] = mkN<N>(s22, i == i0 ? t22 : 0.0);
+      x12[i] = mkN<N>(s12, i == i0 ? t12 : 0.0);
+    }
+  }
+  fixa<PS>(x11, pr, S_X11);
+  fixa<PS>(x22, pr, S_X22);
+  fixa<PS>(x12, pr, S_X12);
+  // Christoffel symbols Γ[v][μ] = x_{,v} · a^μ (v = 11, 22, 12)
+  N G0[3], G1[3];
+  G0[0] = x11[0] * g.au1[0] + x11[1] * g.au1[1] + x11[2] * g.au1[2];
+  G0[1] = x22[0] * g.au1[0] + x22[1] * g.au1[1] + x22[2] * g.au1[2];
+  G0[2] = x12[0] * g.au1[0] + x12[1] * g.au1[1] + x12[2] * g.au1[2];
+  G1[0] = x11[0] * g.au2[0] + x11[1] * g.au2[1] + x11[2] * g.au2[2];
+  G1[1] = x22[0] * g.au2[0] + x22[1] * g.au2[1] + x22[2] * g.au2[2];
+  G1[2] = x12[0] * g.au2[0] + x12[1] * g.au2[1] + x12[2] * g.au2[2];
+  fixa<PS>(G0, pr, S_GA0);
+  fixa<PS>(G1, pr, S_GA1);
+  // curvature b_{αβ}, b^{αβ}, H, K
+  N bb[3];
+  bb[0] = x11[0] * g.n[0] + x11[1] * g.n[1] + x11[2] * g.n[2];
+  bb[1] = x22[0] * g.n[0] + x22[1] * g.n[1] + x22[2] * g.n[2];
+  bb[2] = x12[0] * g.n[0] + x12[1] * g.n[1] + x12[2] * g.n[2];
+  fixa<PS>(bb, pr, S_BB);
+  const N bb11 = bb[0], bb22 = bb[1], bb12 = bb[2];
+  N tb00 = g.g00 * bb11 + g.g01 * bb12, tb01 = g.g00 * bb12 + g.g01 * bb22;
+  N tb10 = g.g01 * bb11 + g.g11 * bb12, tb11 = g.g01 * bb12 + g.g11 * bb22;
+  N Bm[4];
+  Bm[0] = tb00 * g.g00 + tb01 * g.g01;
+  Bm[1] = tb00 * g.g01 + tb01 * g.g11;
+  Bm[2] = tb10 * g.g00 + tb11 * g.g01;
+  Bm[3] = tb10 * g.g01 + tb11 * g.g11;
+  fixa<PS>(Bm, pr, S_B);
+  const N B00 = Bm[0], B01 = Bm[1], B10 = Bm[2], B11 = Bm[3];
+  N H = 0.5 * (g.g00 * bb11 + g.g01 * bb12 + g.g01 * bb12 + g.g11 * bb22);
+  N Kc = (bb11 * bb22 - bb12 * bb12) * g.idet;
+  fix<PS>(H, pr, S_H);
+  fix<PS>(Kc, pr, S_KC);
+
+  // fields: v, v_{,α}, λ, and (EUL/ALE) v^m, v^m_{,α}, p^m
+  N v[3], dv0[3], dv1[3];
+  const double tN = sC0 * bN[a0], tN1 = sC0 * bN1[a0], tN2 = sC0 * bN2[a0];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    double s = 0, s1 = 0, s2 = 0;
+    if (i < L::NV) {  // an absent vz is zero (get_dof_cps, GeoDynStress.jl:190-205)
+#pragma unroll
+      for (int a = 0; a < 9; ++a) {
+        const double c = C[i * 9 + a];
+        s = fma(c, bN[a], s);
+        s1 = fma(c, bN1[a], s1);
+        s2 = fma(c, bN2[a], s2);
+      }
+    }
+    const bool sd = (d0 == i) && i < L::NV;
+    v[i] = mkN<N>(s, sd ? tN : 0.0);
+    dv0[i] = mkN<N>(s1, sd ? tN1 : 0.0);
+    dv1[i] = mkN<N>(s2, sd ? tN2 : 0.0);
+  }
+  fixa<PS>(v, pr, S_V);
+  fixa<PS>(dv0, pr, S_DV0);
+  fixa<PS>(dv1, pr, S_DV1);
+  N lam;
+  {
+    double s = 0;
+#pragma unroll
+    for (int a = 0; a < 9; ++a) s = fma(C[L::LAMI * 9 + a], bN[a], s);
+    lam = mkN<N>(s, d0 == L::LAMI ? tN : 0.0);
+  }
+  fix<PS>(lam, pr, S_LAM);
+  N vm[3], dvm0[3], dvm1[3], pm = mkN<N>(0, 0);
+  if constexpr (L::HAS_VM) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double s = 0, s1 = 0, s2 = 0;
+#pragma unroll
+      for (int a = 0; a < 9; ++a) {
+        const double c = C[(3 + i) * 9 + a];
+        s = fma(c, bN[a], s);
+        s1 = fma(c, bN1[a], s1);
+        s2 = fma(c, bN2[a], s2);
+      }
+      const bool sd = (d0 == 3 + i);
+      vm[i] = mkN<N>(s, sd ? tN : 0.0);
+      dvm0[i] = mkN<N>(s1, sd ? tN1 : 0.0);
+      dvm1[i] = mkN<N>(s2, sd ? tN2 : 0.0);
+    }
+    fixa<PS>(vm, pr, S_VM);
+    fixa<PS>(dvm0, pr, S_DVM0);
+    fixa<PS>(dvm1, pr, S_DVM1);
+  }
+  if constexpr (L::HAS_PM) {
+    double s = 0;
+#pragma unroll
+    for (int a = 0; a < 9; ++a) s = fma(C[L::PMI * 9 + a], bN[a], s);
+    pm = mkN<N>(s, d0 == L::PMI ? tN : 0.0);
+    fix<PS>(pm, pr, S_PM);
+  }
+
+  // stresses (GeoDynStress.jl:148-159), Voigt [11, 22, 2*12] / [11, 22, 12+21]
+  N sg[3], Mv[3], sgm[3];
+  {
+    N w0[3], w1[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      w0[i] = dv0[i] * g.g00 + dv1[i] * g.g01;
+      w1[i] = dv0[i] * g.g01 + dv1[i] * g.g11;
+    }
+    N p00 = g.au1[0] * w0[0] + g.au1[1] * w0[1] + g.au1[2] * w0[2];
+    N p01 = g.au1[0] * w1[0] + g.au1[1] * w1[1] + g.au1[2] * w1[2];
+    N p10 = g.au2[0] * w0[0] + g.au2[1] * w0[1] + g.au2[2] * w0[2];
+    N p11 = g.au2[0] * w1[0] + g.au2[1] * w1[1] + g.au2[2] * w1[2];
+    N c0 = ph.kb * (H * H) - ph.kg * Kc;
+    N lc = c0 + lam;
+    N h2 = (2.0 * ph.kb) * H;
+    sg[0] = g.g00 * lc - B00 * h2 + (2.0 * ph.zv) * p00;
+    sg[1] = g.g11 * lc - B11 * h2 + (2.0 * ph.zv) * p11;
+    sg[2] = 2.0 * (g.g01 * lc - B01 * h2 + ph.zv * (p01 + p10));
+    N kk = H * (ph.kb + 2.0 * ph.kg);
+    Mv[0] = g.g00 * kk - B00 * ph.kg;
+    Mv[1] = g.g11 * kk - B11 * ph.kg;
+    Mv[2] = (g.g01 * kk - B01 * ph.kg) + (g.g01 * kk - B10 * ph.kg);
+    if constexpr (L::HAS_PM) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        w0[i] = dvm0[i] * g.g00 + dvm1[i] * g.g01;
+        w1[i] = dvm0[i] * g.g01 + dvm1[i] * g.g11;
+      }
+      N q00 = g.au1[0] * w0[0] + g.au1[1] * w0[1] + g.au1[2] * w0[2];
+      N q01 = g.au1[0] * w1[0] + g.au1[1] * w1[1] + g.au1[2] * w1[2];
+      N q10 = g.au2[0] * w0[0] + g.au2[1] * w0[1] + g.au2[2] * w0[2];
+      N q11 = g.au2[0] * w1[0] + g.au2[1] * w1[1] + g.au2[2] * w1[2];
+      sgm[0] = g.g00 * c0 - B00 * h2 + (2.0 * ph.zv) * q00;
+      sgm[1] = g.g11 * c0 - B11 * h2 + (2.0 * ph.zv) * q11;
+      sgm[2] = 2.0 * (g.g01 * c0 - B01 * h2 + ph.zv * (q01 + q10));
+      fixa<PS>(sgm, pr, S_SGM);
+    }
+  }
+  fixa<PS>(sg, pr, S_SG);
+  fixa<PS>(Mv, pr, S_MV);
+  if constexpr (PS == PRIMAL) return;
+  else {
+    // per-Gauss-point factors of the residual rows
+    const N wJ = w * g.J;
+    N A1[3], A2[3], Nn[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      A1[i] = wJ * g.a1[i];
+      A2[i] = wJ * g.a2[i];
+      Nn[i] = wJ * g.n[i];
+    }
+    N tr = g.au1[0] * dv0[0] + g.au1[1] * dv0[1] + g.au1[2] * dv0[2] +
+           g.au2[0] * dv1[0] + g.au2[1] * dv1[1] + g.au2[2] * dv1[2];
+    const N ql = w * (g.J * tr - azv * lam);
+    N gm[3], qp = mkN<N>(0, 0), pmJ = mkN<N>(0, 0);
+    if constexpr (MO == EUL) {
+      N nv = g.n[0] * v[0] + g.n[1] * v[1] + g.n[2] * v[2];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) gm[i] = (ph.am * wJ) * (vm[i] - g.n[i] * nv);
+    }
+    if constexpr (L::HAS_PM) {
+      N nd = g.n[0] * (vm[0] - v[0]) + g.n[1] * (vm[1] - v[1]) + g.n[2] * (vm[2] - v[2]);
+      qp = -(wJ * nd) - (w * azv) * pm;
+      pmJ = pm * wJ;
+    }
+
+// local row j NDF + r of this lane is row a NDF + r of the element
+#define ADD(r, val)                                  \
+  do {                                               \
+    const N v_ = (val);                              \
+    acc[j * NDF + (r)] += v_.t;                      \
+    racc += (a * NDF + (r) == col) ? v_.v : 0.0;     \
+  } while (0)
+
+#pragma unroll
+    for (int j = 0; j < NPP; ++j) {
+      const int a = a_lo + j;
+      if (9 % P != 0 && a >= 9) break;
+      const double n1 = bN1[a], n2 = bN2[a], na = bN[a];
+      N s1 = n1 * sg[0] + (0.5 * n2) * sg[2];
+      N s2 = n2 * sg[1] + (0.5 * n1) * sg[2];
+      N nn0 = mkN<N>(b11[a], 0.0) - (n1 * G0[0] + n2 * G1[0]);
+      N nn1 = mkN<N>(b22[a], 0.0) - (n1 * G0[1] + n2 * G1[1]);
+      N nn2 = mkN<N>(b12[a], 0.0) - (n1 * G0[2] + n2 * G1[2]);
+      N m = nn0 * Mv[0] + nn1 * Mv[1] + nn2 * Mv[2];
+#pragma unroll
+      for (int i = 0; i < L::NV; ++i) {
+        N rv = A1[i] * s1 + A2[i] * s2 + Nn[i] * m;
+        if (ph.pn != 0.0) rv = rv - (ph.pn * na) * Nn[i];
+        ADD(i, rv);
+      }
+      ADD(L::LAMI, na * ql);
+      if constexpr (MO == EUL) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) ADD(3 + i, na * gm[i]);
+      }
+      if constexpr (L::HAS_PM) {
+        N t1 = n1 * sgm[0] + (0.5 * n2) * sgm[2];
+        N t2 = n2 * sgm[1] + (0.5 * n1) * sgm[2];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          N rm = A1[i] * t1 + A2[i] * t2;
+          if constexpr (MO == ALEVB) rm = rm + Nn[i] * m;
+          rm = rm - na * (pmJ * g.n[i]);
+          ADD(3 + i, rm);
+        }
+        ADD(L::PMI, na * qp);
+      }
+    }
+#undef ADD
+  }
+}
+
+// LIST = false: elements 0..numel-1, outputs K_el and r_el per element, in
+// two passes (PRIMAL, then TANGENT).  LIST = true: the numel elements
+// elist[0..numel-1], output r_el only, at the element's position in the list
+// (PLAIN: the tangents are dead code there).  P lanes share each tangent
+// column (lane part p takes the rows of nodes p NPP .. p NPP + NPP - 1, NPP =
+// ceil(9 / P)); the part holding the column's own row writes its r_el entry.
 template <int MO, bool LIST = false, int P = LIST ? 1 : Lay<MO>::PARTS>
 __device__ __forceinline__ void area_body(
     int64_t bid, int64_t numel, int64_t numnp, const int32_t *__restrict__ ix,
@@ -115,9 +429,12 @@ __device__ __forceinline__ void area_body(
   constexpr int NDF = L::NDF, NE = L::NE;
   constexpr int NPP = (9 + P - 1) / P, NR = NPP * NDF, LPE = NE * P;
   constexpr int MAXE = 256 / LPE + 2;  // elements a 256-lane block can touch
+  constexpr bool STAGED = !LIST;
+  constexpr int NS = L::HAS_VM ? N_SLOTS : S_VM;  // slots this motion uses
   __shared__ double sX[MAXE][3 * 9];
   __shared__ double sC[MAXE][NDF * 9];
   __shared__ double sB[MAXE][9 * RECP];
+  __shared__ double sP[STAGED ? MAXE * 9 : 1][STAGED ? NS : 1];
 
   const int64_t f0 = bid * 256;
   const int64_t e_lo = f0 / LPE;
@@ -141,6 +458,19 @@ __device__ __forceinline__ void area_body(
   }
   __syncthreads();
 
+  const double azv = ph.adb / ph.zv;
+  if constexpr (STAGED) {
+    // primal pass: one lane per (element, Gauss point) of the block
+    for (int k = threadIdx.x; k < nle * 9; k += 256) {
+      const int le = k / 9, gp = k % 9;
+      double none[1];
+      double rnone = 0.0;
+      gp_eval<MO, R, PRIMAL, 1, 1>(sX[le], sC[le], &sB[le][gp * RECP], 0, -1, -1, 0, -1, 0.0,
+                                   0.0, ph, azv, sP[k], none, rnone);
+    }
+    __syncthreads();
+  }
+
   const int64_t f = f0 + threadIdx.x;
   const bool valid = f < numel * LPE;
   const int64_t e = valid ? f / LPE : e_hi;
@@ -157,219 +487,20 @@ __device__ __forceinline__ void area_body(
   const bool mot = (L::MOT0 >= 0) && d0 >= L::MOT0 && d0 < L::MOT0 + 3;
   const int i0 = mot ? d0 - L::MOT0 : -1;
   const double sX0 = (act && mot) ? dt : 0.0;
-  const double *X = sX[le];
   const double *C = sC[le];
 
   double acc[NR];
 #pragma unroll
   for (int k = 0; k < NR; ++k) acc[k] = 0.0;
   double racc = 0.0;
-  const double azv = ph.adb / ph.zv;
 
   for (int gp = 0; gp < 9; ++gp) {
-    const double *rec = &sB[le][gp * RECP];
-    const double w = rec[0];
-    const double *bN = rec + 1, *bN1 = rec + 10, *bN2 = rec + 19;
-    const double *b11 = rec + 28, *b22 = rec + 37, *b12 = rec + 46;
-
-    Geo g;
-    geometry(X, bN1, bN2, a0, i0, sX0, g);
-    // second derivatives of position: x_{,11}, x_{,22}, x_{,12}
-    D x11[3], x22[3], x12[3];
-    {
-      const double t11 = sX0 * b11[a0], t22 = sX0 * b22[a0], t12 = sX0 * b12[a0];
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        double s11 = 0, s22 = 0, s12 = 0;
-#pragma unroll
-        for (int a = 0; a < 9; ++a) {
-          s11 = fma(X[i * 9 + a], b11[a], s11);
-          s22 = fma(X[i * 9 + a], b22[a], s22);
-          s12 = fma(X[i * 9 + a], b12[a], s12);
-        }
-        x11[i] = mk(s11, i == i0 ? t11 : 0.0);
-        x22[i] = mk(s22, i == i0 ? t22 : 0.0);
-        x12[i] = mk(s12, i == i0 ? t12 : 0.0);
-      }
-    }
-    // Christoffel symbols Γ[v][μ] = x_{,v} · a^μ (v = 11, 22, 12)
-    D G0[3], G1[3];
-    G0[0] = x11[0] * g.au1[0] + x11[1] * g.au1[1] + x11[2] * g.au1[2];
-    G0[1] = x22[0] * g.au1[0] + x22[1] * g.au1[1] + x22[2] * g.au1[2];
-    G0[2] = x12[0] * g.au1[0] + x12[1] * g.au1[1] + x12[2] * g.au1[2];
-    G1[0] = x11[0] * g.au2[0] + x11[1] * g.au2[1] + x11[2] * g.au2[2];
-    G1[1] = x22[0] * g.au2[0] + x22[1] * g.au2[1] + x22[2] * g.au2[2];
-    G1[2] = x12[0] * g.au2[0] + x12[1] * g.au2[1] + x12[2] * g.au2[2];
-    // curvature b_{αβ}, b^{αβ}, H, K
-    D bb11 = x11[0] * g.n[0] + x11[1] * g.n[1] + x11[2] * g.n[2];
-    D bb22 = x22[0] * g.n[0] + x22[1] * g.n[1] + x22[2] * g.n[2];
-    D bb12 = x12[0] * g.n[0] + x12[1] * g.n[1] + x12[2] * g.n[2];
-    D tb00 = g.g00 * bb11 + g.g01 * bb12, tb01 = g.g00 * bb12 + g.g01 * bb22;
-    D tb10 = g.g01 * bb11 + g.g11 * bb12, tb11 = g.g01 * bb12 + g.g11 * bb22;
-    D B00 = tb00 * g.g00 + tb01 * g.g01, B01 = tb00 * g.g01 + tb01 * g.g11;
-    D B10 = tb10 * g.g00 + tb11 * g.g01, B11 = tb10 * g.g01 + tb11 * g.g11;
-    D H = 0.5 * (g.g00 * bb11 + g.g01 * bb12 + g.g01 * bb12 + g.g11 * bb22);
-    D Kc = (bb11 * bb22 - bb12 * bb12) * g.idet;
-
-    // fields: v, v_{,α}, λ, and (EUL/ALE) v^m, v^m_{,α}, p^m
-    D v[3], dv0[3], dv1[3];
-    const double tN = sC0 * bN[a0], tN1 = sC0 * bN1[a0], tN2 = sC0 * bN2[a0];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      double s = 0, s1 = 0, s2 = 0;
-      if (i < L::NV) {  // an absent vz is zero (get_dof_cps, GeoDynStress.jl:190-205)
-#pragma unroll
-        for (int a = 0; a < 9; ++a) {
-          const double c = C[i * 9 + a];
-          s = fma(c, bN[a], s);
-          s1 = fma(c, bN1[a], s1);
-          s2 = fma(c, bN2[a], s2);
-        }
-      }
-      const bool sd = (d0 == i) && i < L::NV;
-      v[i] = mk(s, sd ? tN : 0.0);
-      dv0[i] = mk(s1, sd ? tN1 : 0.0);
-      dv1[i] = mk(s2, sd ? tN2 : 0.0);
-    }
-    D lam;
-    {
-      double s = 0;
-#pragma unroll
-      for (int a = 0; a < 9; ++a) s = fma(C[L::LAMI * 9 + a], bN[a], s);
-      lam = mk(s, d0 == L::LAMI ? tN : 0.0);
-    }
-    D vm[3], dvm0[3], dvm1[3], pm = mk(0, 0);
-    if constexpr (L::HAS_VM) {
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        double s = 0, s1 = 0, s2 = 0;
-#pragma unroll
-        for (int a = 0; a < 9; ++a) {
-          const double c = C[(3 + i) * 9 + a];
-          s = fma(c, bN[a], s);
-          s1 = fma(c, bN1[a], s1);
-          s2 = fma(c, bN2[a], s2);
-        }
-        const bool sd = (d0 == 3 + i);
-        vm[i] = mk(s, sd ? tN : 0.0);
-        dvm0[i] = mk(s1, sd ? tN1 : 0.0);
-        dvm1[i] = mk(s2, sd ? tN2 : 0.0);
-      }
-    }
-    if constexpr (L::HAS_PM) {
-      double s = 0;
-#pragma unroll
-      for (int a = 0; a < 9; ++a) s = fma(C[L::PMI * 9 + a], bN[a], s);
-      pm = mk(s, d0 == L::PMI ? tN : 0.0);
-    }
-
-    // stresses (GeoDynStress.jl:148-159), Voigt [11, 22, 2*12] / [11, 22, 12+21]
-    D sg[3], Mv[3], sgm[3];
-    {
-      D w0[3], w1[3];
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        w0[i] = dv0[i] * g.g00 + dv1[i] * g.g01;
-        w1[i] = dv0[i] * g.g01 + dv1[i] * g.g11;
-      }
-      D p00 = g.au1[0] * w0[0] + g.au1[1] * w0[1] + g.au1[2] * w0[2];
-      D p01 = g.au1[0] * w1[0] + g.au1[1] * w1[1] + g.au1[2] * w1[2];
-      D p10 = g.au2[0] * w0[0] + g.au2[1] * w0[1] + g.au2[2] * w0[2];
-      D p11 = g.au2[0] * w1[0] + g.au2[1] * w1[1] + g.au2[2] * w1[2];
-      D c0 = ph.kb * (H * H) - ph.kg * Kc;
-      D lc = c0 + lam;
-      D h2 = (2.0 * ph.kb) * H;
-      sg[0] = g.g00 * lc - B00 * h2 + (2.0 * ph.zv) * p00;
-      sg[1] = g.g11 * lc - B11 * h2 + (2.0 * ph.zv) * p11;
-      sg[2] = 2.0 * (g.g01 * lc - B01 * h2 + ph.zv * (p01 + p10));
-      D kk = H * (ph.kb + 2.0 * ph.kg);
-      Mv[0] = g.g00 * kk - B00 * ph.kg;
-      Mv[1] = g.g11 * kk - B11 * ph.kg;
-      Mv[2] = (g.g01 * kk - B01 * ph.kg) + (g.g01 * kk - B10 * ph.kg);
-      if constexpr (L::HAS_PM) {
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          w0[i] = dvm0[i] * g.g00 + dvm1[i] * g.g01;
-          w1[i] = dvm0[i] * g.g01 + dvm1[i] * g.g11;
-        }
-        D q00 = g.au1[0] * w0[0] + g.au1[1] * w0[1] + g.au1[2] * w0[2];
-        D q01 = g.au1[0] * w1[0] + g.au1[1] * w1[1] + g.au1[2] * w1[2];
-        D q10 = g.au2[0] * w0[0] + g.au2[1] * w0[1] + g.au2[2] * w0[2];
-        D q11 = g.au2[0] * w1[0] + g.au2[1] * w1[1] + g.au2[2] * w1[2];
-        sgm[0] = g.g00 * c0 - B00 * h2 + (2.0 * ph.zv) * q00;
-        sgm[1] = g.g11 * c0 - B11 * h2 + (2.0 * ph.zv) * q11;
-        sgm[2] = 2.0 * (g.g01 * c0 - B01 * h2 + ph.zv * (q01 + q10));
-      }
-    }
-
-    // per-Gauss-point factors of the residual rows
-    const D wJ = w * g.J;
-    D A1[3], A2[3], Nn[3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      A1[i] = wJ * g.a1[i];
-      A2[i] = wJ * g.a2[i];
-      Nn[i] = wJ * g.n[i];
-    }
-    D tr = g.au1[0] * dv0[0] + g.au1[1] * dv0[1] + g.au1[2] * dv0[2] +
-           g.au2[0] * dv1[0] + g.au2[1] * dv1[1] + g.au2[2] * dv1[2];
-    const D ql = w * (g.J * tr - azv * lam);
-    D gm[3], qp = mk(0, 0), pmJ = mk(0, 0);
-    if constexpr (MO == EUL) {
-      D nv = g.n[0] * v[0] + g.n[1] * v[1] + g.n[2] * v[2];
-#pragma unroll
-      for (int i = 0; i < 3; ++i) gm[i] = (ph.am * wJ) * (vm[i] - g.n[i] * nv);
-    }
-    if constexpr (L::HAS_PM) {
-      D nd = g.n[0] * (vm[0] - v[0]) + g.n[1] * (vm[1] - v[1]) + g.n[2] * (vm[2] - v[2]);
-      qp = -(wJ * nd) - (w * azv) * pm;
-      pmJ = pm * wJ;
-    }
-
-// local row j NDF + r of this lane is row a NDF + r of the element
-#define ADD(r, val)                                  \
-  do {                                               \
-    const D v_ = (val);                              \
-    acc[j * NDF + (r)] += v_.t;                      \
-    racc += (a * NDF + (r) == col) ? v_.v : 0.0;     \
-  } while (0)
-
-#pragma unroll
-    for (int j = 0; j < NPP; ++j) {
-      const int a = a_lo + j;
-      if (9 % P != 0 && a >= 9) break;
-      const double n1 = bN1[a], n2 = bN2[a], na = bN[a];
-      D s1 = n1 * sg[0] + (0.5 * n2) * sg[2];
-      D s2 = n2 * sg[1] + (0.5 * n1) * sg[2];
-      D nn0 = mk(b11[a], 0.0) - (n1 * G0[0] + n2 * G1[0]);
-      D nn1 = mk(b22[a], 0.0) - (n1 * G0[1] + n2 * G1[1]);
-      D nn2 = mk(b12[a], 0.0) - (n1 * G0[2] + n2 * G1[2]);
-      D m = nn0 * Mv[0] + nn1 * Mv[1] + nn2 * Mv[2];
-#pragma unroll
-      for (int i = 0; i < L::NV; ++i) {
-        D rv = A1[i] * s1 + A2[i] * s2 + Nn[i] * m;
-        if (ph.pn != 0.0) rv = rv - (ph.pn * na) * Nn[i];
-        ADD(i, rv);
-      }
-      ADD(L::LAMI, na * ql);
-      if constexpr (MO == EUL) {
-#pragma unroll
-        for (int i = 0; i < 3; ++i) ADD(3 + i, na * gm[i]);
-      }
-      if constexpr (L::HAS_PM) {
-        D t1 = n1 * sgm[0] + (0.5 * n2) * sgm[2];
-        D t2 = n2 * sgm[1] + (0.5 * n1) * sgm[2];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          D rm = A1[i] * t1 + A2[i] * t2;
-          if constexpr (MO == ALEVB) rm = rm + Nn[i] * m;
-          rm = rm - na * (pmJ * g.n[i]);
-          ADD(3 + i, rm);
-        }
-        ADD(L::PMI, na * qp);
-      }
-    }
-#undef ADD
+    if constexpr (STAGED)
+      gp_eval<MO, D, TANGENT, P>(sX[le], C, &sB[le][gp * RECP], a0, d0, i0, a_lo, col, sX0,
+                                   sC0, ph, azv, sP[le * 9 + gp], acc, racc);
+    else
+      gp_eval<MO, D, PLAIN, P>(sX[le], C, &sB[le][gp * RECP], a0, d0, i0, a_lo, col, sX0, sC0,
+                                 ph, azv, nullptr, acc, racc);
   }
 
   // Dohrmann--Bochev projection (FiniteElement.jl:315-327): linear in λ, p^m
@@ -461,7 +592,7 @@ __device__ __forceinline__ void edge_body(
     const double *rec = &sB[le][gp * RECP];
     const double w = rec[0];
     const double *bN = rec + 1, *bN1 = rec + 10, *bN2 = rec + 19;
-    Geo g;
+    Geo<> g;
     geometry(X, bN1, bN2, a0, i0, sX0, g);
     // edge frame (calc_τ_ν): τ = ±a_1 / ±a_2, normalised by the conjugating
     // dot product, whose complex-step derivative is zero: only τ's own
@@ -521,14 +652,15 @@ __device__ __forceinline__ void edge_body(
   }
 }
 
-template <int MO>
+template <int MO, int P = Lay<MO>::PARTS>
 __global__ __launch_bounds__(256) void k_area(
     int64_t numel, int64_t numnp, const int32_t *__restrict__ ix,
     const int32_t *__restrict__ uel, const uint8_t *__restrict__ active,
     const double *__restrict__ xms, const double *__restrict__ cps,
     const double *__restrict__ ufns, const double *__restrict__ dbt, PhysParams ph,
     double dt, double *__restrict__ kel, double *__restrict__ rel) {
-  area_body<MO>(blockIdx.x, numel, numnp, ix, uel, active, xms, cps, ufns, dbt, ph, dt, kel, rel);
+  area_body<MO, false, P>(blockIdx.x, numel, numnp, ix, uel, active, xms, cps, ufns, dbt, ph, dt,
+                          kel, rel);
 }
 
 template <int MO>

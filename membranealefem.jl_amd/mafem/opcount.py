@@ -13,9 +13,9 @@ dual scalar that tallies +, -, *, /, sqrt (an FMA is two), on an interior
 element of the 8 x 8 F_PULL patch; the count has no data-dependent branch
 (for pn = 0) and is the same for every direction.
 
-Algorithmic bytes: the element state 9 (3 + ndf) doubles, r_el 9 ndf doubles,
-K_el read-modify-written once (2 (9 ndf)^2 doubles) and the 9 node indices
-read twice (4 bytes each); basis tables are cache-resident and not counted.
+Algorithmic bytes: the element state 9 (3 + ndf) doubles read, r_el 9 ndf
+doubles and K_el (9 ndf)^2 doubles written once, and the 9 node indices read
+twice (4 bytes each); basis tables are cache-resident and not counted.
 """
 
 # motion -> (F_res, F_dir, n_dir of an interior element)
@@ -52,4 +52,4 @@ def elem_bytes(motion):
     """Algorithmic HBM bytes of one elem-assembly."""
     ndf = NDF[motion]
     ne = 9 * ndf
-    return 9 * (3 + ndf) * 8 + ne * 8 + 2 * ne * ne * 8 + 2 * 9 * 4
+    return 9 * (3 + ndf) * 8 + ne * 8 + ne * ne * 8 + 2 * 9 * 4
