@@ -457,7 +457,7 @@ __device__ __forceinline__ void tile_mm(const S *__restrict__ a, const S *__rest
 constexpr int MM1_LDS = T * (T + 1) + 2 * T;  // B (or a staged tile) + two index arrays
 template <class S>
 __device__ __forceinline__ void tile_mm_ga(const S *__restrict__ a, const S *__restrict__ b,
-                                           S c[4][4], S *lds) {
+                                           S c[4][4], S *lds, const int *brow = nullptr) {
   using V = typename Mfma<S>::V;
   constexpr int P = T + 1;
   constexpr int W = 16 / sizeof(S);  // entries per 16-byte load
@@ -470,8 +470,11 @@ __device__ __forceinline__ void tile_mm_ga(const S *__restrict__ a, const S *__r
   for (int q = 0; q < 16 / W; ++q) ra[q] = *(const Vec *)(ag + q * W);
   Vec vb[T * T / W / 256];
 #pragma unroll
-  for (int it = 0; it < T * T / W / 256; ++it)
-    vb[it] = *(const Vec *)(b + (it * 256 + (int)threadIdx.x) * W);
+  for (int it = 0; it < T * T / W / 256; ++it) {
+    const int e = (it * 256 + (int)threadIdx.x) * W;
+    // brow: row r of the B operand is row brow[r] of b
+    vb[it] = *(const Vec *)(brow ? b + brow[e / T] * T + e % T : b + e);
+  }
 #pragma unroll
   for (int it = 0; it < T * T / W / 256; ++it) {
     const int e = (it * 256 + (int)threadIdx.x) * W;
@@ -705,6 +708,157 @@ __device__ __forceinline__ void diag_mm_lds(int64_t K, S *__restrict__ A, int32_
   S(*st)[T + 1] = reinterpret_cast<S(*)[T + 1]>(lds);
   int *prow = reinterpret_cast<int *>(lds + T * (T + 1));
   diag_core_mm<S, PERT>(K, A, piv, st, prow, prow + T, amax, pub);
+}
+
+// In-place inverses of the factors of a 64x64 diagonal tile factored by
+// diag_core_mm (row i of P A = L U at st[prow[i]]: L unit lower below the
+// diagonal, U upper), in 16 x 16 blocks.  The rows are first put in pivot
+// order.  The diagonal blocks by column sweeps in place, one row per lane
+// (wavefronts 0-1: L_bb^-1 of blocks 2w and 2w + 1, wavefronts 2-3: U_bb^-1),
+// then three rounds on the matrix cores: round r forms block row r + 1 of
+// X = L^-1, X_ij = -X_ii sum_{k=j}^{i-1} L_ik X_kj, and block row 2 - r of
+// Y = U^-1, Y_ij = -Y_ii sum_{k=i+1}^{j} U_ik Y_kj, each from blocks that are
+// final (every sum is formed before any job overwrites its block).  Li and Ui
+// receive the inverses as full row-major tiles (zeros outside the triangles,
+// ones on L^-1's diagonal).  256 threads; st is overwritten.
+template <class S>
+__device__ __forceinline__ int k_of(int kk) {  // the k a lane group feeds in MFMA step kk
+  return acc_row<S>(kk);
+}
+template <class S>
+__device__ __forceinline__ void tri_inv_lds(S (*st)[T + 1], const int *prow, S *__restrict__ Li,
+                                            S *__restrict__ Ui) {
+  using V = typename Mfma<S>::V;
+  const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+  const int cc = lane & 15;
+  {  // rows into pivot order
+    S v[T * T / 256];
+#pragma unroll
+    for (int it = 0; it < T * T / 256; ++it) {
+      const int e = it * 256 + (int)threadIdx.x;
+      v[it] = st[prow[e / T]][e % T];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < T * T / 256; ++it) {
+      const int e = it * 256 + (int)threadIdx.x;
+      st[e / T][e % T] = v[it];
+    }
+    __syncthreads();
+  }
+  if (lane < 32) {
+    // in place, row i = lane % 16 of block b per lane, columns in sequence
+    // (a step's reads come before its writes: one wavefront per block pair)
+    const int b0 = 16 * (2 * (w & 1) + (lane >> 4)), i = cc;
+    S *ri = &st[b0 + i][b0];
+    if (w < 2) {  // X = L_bb^-1, columns from the last: X[i][j] = -sum_{k=j+1}^{i} X[i][k] L[k][j]
+#pragma unroll 1
+      for (int j = 14; j >= 0; --j) {
+        // every operand is loaded (in bounds), the terms outside j < k < i
+        // enter as zeros: the loads issue together
+        S a[14], b[14];
+#pragma unroll
+        for (int k = 1; k < 15; ++k) {
+          a[k - 1] = ri[k];
+          b[k - 1] = st[b0 + k][b0 + j];
+        }
+        const S lij = ri[j];
+        S acc0 = i > j ? lij : S(0), acc1 = S(0);  // k = i: X[i][i] = 1
+#pragma unroll
+        for (int k = 1; k < 15; ++k) {
+          const S t = (k > j && k < i) ? a[k - 1] : S(0);
+          if (k & 1) acc1 = fma(t, b[k - 1], acc1);
+          else acc0 = fma(t, b[k - 1], acc0);
+        }
+        if (i > j) ri[j] = -(acc0 + acc1);
+      }
+    } else {  // Y = U_bb^-1, columns from the first: Y[i][j] = -Y[j][j] sum_{k=i}^{j-1} Y[i][k] U[k][j]
+#pragma unroll 1
+      for (int j = 0; j < 16; ++j) {
+        S a[15], b[15];
+#pragma unroll
+        for (int k = 0; k < 15; ++k) {
+          a[k] = ri[k];
+          b[k] = st[b0 + k][b0 + j];
+        }
+        const S yjj = S(1) / st[b0 + j][b0 + j];
+        S acc0 = S(0), acc1 = S(0);
+#pragma unroll
+        for (int k = 0; k < 15; ++k) {
+          const S t = (k >= i && k < j) ? a[k] : S(0);
+          if (k & 1) acc1 = fma(t, b[k], acc1);
+          else acc0 = fma(t, b[k], acc0);
+        }
+        if (i < j) ri[j] = -(acc0 + acc1) * yjj;
+        else if (i == j) ri[j] = yjj;
+      }
+    }
+  }
+  __syncthreads();
+  // entry (r, q) of block (bi, bj) of the tile
+  auto at = [&](int bi, int bj, int r, int q) -> S { return st[16 * bi + r][16 * bj + q]; };
+#pragma unroll 1
+  for (int rd = 0; rd < 3; ++rd) {
+    const int njob = 2 * (rd + 1);
+    V tacc[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int q = w + 4 * h;
+      tacc[h] = V{0, 0, 0, 0};
+      if (q >= njob) continue;
+      // the diagonal block of the sum (X_jj unit lower, Y_jj upper) is read
+      // whole and masked
+      if (q <= rd) {  // X row i = rd + 1, column j = q: sum_{k=j}^{i-1} L_ik X_kj
+        const int i = rd + 1, j = q;
+        for (int k = j; k < i; ++k)
+#pragma unroll
+          for (int kk = 0; kk < 4; ++kk) {
+            const int km = k_of<S>(kk);
+            const S a = at(i, k, cc, km), bv = at(k, j, km, cc);
+            const S b = k > j || km > cc ? bv : (km == cc ? S(1) : S(0));
+            tacc[h] = Mfma<S>::run(a, b, tacc[h]);
+          }
+      } else {  // Y row i = 2 - rd, column j = i + 1 + (q - rd - 1): sum_{k=i+1}^{j} U_ik Y_kj
+        const int i = 2 - rd, j = i + 1 + (q - rd - 1);
+        for (int k = i + 1; k <= j; ++k)
+#pragma unroll
+          for (int kk = 0; kk < 4; ++kk) {
+            const int km = k_of<S>(kk);
+            const S a = at(i, k, cc, km), bv = at(k, j, km, cc);
+            const S b = k < j || km <= cc ? bv : S(0);
+            tacc[h] = Mfma<S>::run(a, b, tacc[h]);
+          }
+      }
+    }
+    __syncthreads();  // every sum is formed before any block of this round is overwritten
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int q = w + 4 * h;
+      if (q >= njob) continue;
+      const bool isx = q <= rd;
+      const int i = isx ? rd + 1 : 2 - rd, j = isx ? q : i + 1 + (q - rd - 1);
+      V acc = V{0, 0, 0, 0};
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+        const int km = k_of<S>(kk);
+        // -X_ii (unit lower) or -Y_ii (upper); the sum's accumulator row
+        // k_of(kk) of this lane's column is the B operand of step kk
+        const S d = at(i, i, cc, km);
+        const S a = isx ? (cc > km ? -d : (cc == km ? S(-1) : S(0))) : (cc <= km ? -d : S(0));
+        acc = Mfma<S>::run(a, tacc[h][kk], acc);
+      }
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj) st[16 * i + acc_row<S>(jj)][16 * j + cc] = acc[jj];
+    }
+    __syncthreads();
+  }
+#pragma unroll 4
+  for (int it = 0; it < T * T / 256; ++it) {
+    const int e = it * 256 + (int)threadIdx.x, r = e / T, c = e % T;
+    const S v = st[r][c];
+    Li[e] = r > c ? v : (r == c ? S(1) : S(0));
+    Ui[e] = r <= c ? v : S(0);
+  }
 }
 
 // diagonal LU of tile (K, K) of a band by a 256-thread block
