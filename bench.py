@@ -59,6 +59,7 @@ def pull_speed(k, total=200, final=2.0):
 STEP_COUNTERS_KIB = {
     "k_step_la": (16013.467625473484, 19804.575757575756),
     "knd_step_la": (5688.40875, 5070.185),
+    "knd_schur": (209636.8, 93548.0),
 }
 # the same counters per dispatch of the area kernel of the bench patch, by motion
 AREA_COUNTERS_KIB = {
@@ -744,7 +745,8 @@ def main():
     # the bottom part upward) in each launch; the separator's tb steps follow
     # as single-chain launches.
     if args.ordering == "nd":
-        res["roofline"] = nd_roofline(step_stats, agg)
+        res["roofline"] = nd_schur_roofline(step_stats, agg)
+        res["step_roofline"] = nd_roofline(step_stats, agg)
         finish_single(args, res, agg, mesh, numel, p, xms, cps, dev)
         return
     _, tb = dev_band_shape(dev)
@@ -785,6 +787,38 @@ def main():
                                                              max(1, agg["n_factor"])) / 1e12
                        if agg["factor_us"] > 0 else None}
     finish_single(args, res, agg, mesh, numel, p, xms, cps, dev)
+
+
+def nd_schur_roofline(st_list, agg):
+    """The nested-dissection factorization's matrix-core kernel, knd_schur
+    (one launch per tree level: every front's boundary block minus the
+    products of its fully-summed tile rows and columns, accumulated on the
+    matrix cores and written once), which performs about three quarters of
+    the factorization's FLOPs.  The library times every launch on the device's
+    constant clock and counts its algorithmic FLOPs ((FT - fT)^2 fT tile
+    products of 2*64^3 per front) and bytes; achieved = mean FLOPs per launch /
+    mean launch duration over the timed steps."""
+    w = [s for s in st_list if s["schur_kernel_us"] > 0]
+    us = float(np.mean([s["schur_kernel_us"] for s in w])) if w else float("nan")
+    fl = float(np.mean([s["schur_flops"] for s in w])) if w else float("nan")
+    by = float(np.mean([s["schur_bytes"] for s in w])) if w else float("nan")
+    achieved = fl / (us * 1e-6) / 1e12
+    traffic = None
+    counters = STEP_COUNTERS_KIB.get("knd_schur")
+    if counters is not None:
+        traffic = (counters[0] * (131072.0 / 65545.0) + counters[1] * 1.0) * 1024.0
+    st = st_list[-1]
+    per_fact_us = agg["factor_us"] / max(1, agg["n_factor"])
+    return {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+            "traffic_unit": "bytes per launch", "algorithmic_bytes": by,
+            "kernel": ("knd_schur (the Schur blocks of every front of a nested-dissection "
+                       "level: boundary tiles minus the fully-summed rows' and columns' "
+                       "products, FP64 on the matrix cores)"),
+            "flops_per_launch": fl, "avg_us": us,
+            "factorization_tflops": st["front_flops"] / (per_fact_us * 1e-6) / 1e12,
+            "flops_per_factorization": st["front_flops"], "fronts": st["fronts"],
+            "dependent_tile_steps": st["front_steps"]}
 
 
 def nd_roofline(st_list, agg):

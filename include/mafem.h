@@ -137,6 +137,16 @@ typedef struct {
                                       averages, per launch (same operation counts) */
   double step_bytes;               /* MAFEM_ORDER_ND: their algorithmic bytes per launch
                                       (tiles read and written once; else 0) */
+  double schur_kernel_us;          /* MAFEM_ORDER_ND: mean duration of one tree level's
+                                      Schur-block launch (every front's boundary block
+                                      minus its fully-summed products), first block
+                                      start to last block end on the device's constant
+                                      clock (else 0) */
+  double schur_flops;              /* algorithmic FLOPs per such launch (per front
+                                      (FT - fT)^2 fT tile products of 2*64^3) */
+  double schur_bytes;              /* algorithmic bytes per such launch (each operand
+                                      tile read once, each output tile read and written
+                                      once) */
 } mafem_stats;
 
 /* Builds the device context: uploads the tables, builds the block-sparse

@@ -974,7 +974,7 @@ int mafem_time_step(mafem_ctx *ctx, double time, double dt, int32_t max_iter, in
     int it = 0;
     const int32_t nbs0 = c.n_band_solve, nf0 = c.n_factor, ng0 = c.n_gmres;
     const int32_t np0 = read_pert(c);
-    int n_step = 0;
+    int n_step = 0, n_sch = 0;
     double step_fl = 0.0, step_by = 0.0;
     c.step_timed = false;
     MAFEM_HIP(hipEventRecord(c.ev[0], c.stream));
@@ -1005,14 +1005,17 @@ int mafem_time_step(mafem_ctx *ctx, double time, double dt, int32_t max_iter, in
         }
         c.step_timed = false;
       }
-      {
-        double us = 0.0, fl = 0.0, by = 0.0;
-        int cnt = 0;
-        if (c.ordering == 1 && nd_read_spans(c, &us, &cnt, &fl, &by)) {
-          st.step_kernel_us += us;
-          n_step += cnt;
-          step_fl += fl;
-          step_by += by;
+      if (c.ordering == 1) {
+        NdSpans sa, sb;
+        if (nd_read_spans(c, sa, sb)) {
+          st.step_kernel_us += sa.us;
+          n_step += sa.count;
+          step_fl += sa.flops;
+          step_by += sa.bytes;
+          st.schur_kernel_us += sb.us;
+          n_sch += sb.count;
+          st.schur_flops += sb.flops;
+          st.schur_bytes += sb.bytes;
         }
       }
       st.last_relres = c.stats.last_relres;
@@ -1033,6 +1036,11 @@ int mafem_time_step(mafem_ctx *ctx, double time, double dt, int32_t max_iter, in
       st.step_kernel_us /= n_step;
       st.step_flops = step_fl / n_step;
       st.step_bytes = step_by / n_step;
+    }
+    if (n_sch > 0) {
+      st.schur_kernel_us /= n_sch;
+      st.schur_flops /= n_sch;
+      st.schur_bytes /= n_sch;
     }
     st.sweep_groups = c.sweep_groups;
     st.pivot_perturbations = read_pert(c) - np0;

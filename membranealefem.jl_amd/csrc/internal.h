@@ -356,7 +356,11 @@ int nd_fronts(const Ctx &c);     // fronts of the elimination tree (0: not built
 int nd_steps(const Ctx &c);      // dependent tile steps of one factorization
 double nd_flops(const Ctx &c);   // algorithmic FLOPs of one factorization
 // durations (sum, us) / count / FLOPs / bytes of the last factorization's step launches
-bool nd_read_spans(Ctx &c, double *us, int *count, double *flops, double *bytes);
+struct NdSpans {
+  double us = 0, flops = 0, bytes = 0;  // sums over the timed launches
+  int count = 0;
+};
+bool nd_read_spans(Ctx &c, NdSpans &step, NdSpans &schur);
 int64_t nd_bytes(const Ctx &c);  // bytes of factor storage
 void bsr_spmv(Ctx &c, const double *d_x, double *d_y);
 void vec_axpby(Ctx &c, int64_t n, double a, const double *x, double b, double *y);

@@ -59,7 +59,8 @@ class Stats(ctypes.Structure):
         ("factor_chains", ctypes.c_int32), ("separator_rows", ctypes.c_int32),
         ("fronts", ctypes.c_int32), ("front_steps", ctypes.c_int32),
         ("front_flops", ctypes.c_double), ("step_flops", ctypes.c_double),
-        ("step_bytes", ctypes.c_double),
+        ("step_bytes", ctypes.c_double), ("schur_kernel_us", ctypes.c_double),
+        ("schur_flops", ctypes.c_double), ("schur_bytes", ctypes.c_double),
     ]
 
 
