@@ -51,19 +51,20 @@ def pull_speed(k, total=200, final=2.0):
     return final * (k + 1) / total
 
 
-# HBM counters per dispatch of the factorization-step kernel (FETCH_SIZE,
-# WRITE_SIZE in KiB, mean over one time step's dispatches of the 64x64 ALEVB
-# bench patch), by kernel; see the rooflines below.  None: not measured.
+# HBM counters per dispatch of the factorization kernels the rooflines report
+# (FETCH_SIZE, WRITE_SIZE in KiB, mean over one time step's dispatches of the
+# 64x64 ALEVB bench patch), by kernel; see the rooflines below.  None: not
+# measured.
 # profiles/k_step_la_hbm_traffic_pmc.txt and profiles/nd_hbm_traffic_pmc.txt
 # hold the runs.
 STEP_COUNTERS_KIB = {
     "k_step_la": (16013.467625473484, 19804.575757575756),
-    "knd_step_la": (5688.40875, 5070.185),
-    "knd_schur": (209636.8, 93548.0),
+    "knd_step_la": (3678.3363970588234, 3150.5367647058824),
+    "knd_schur": (209711.966796875, 93548.0),
 }
 # the same counters per dispatch of the area kernel of the bench patch, by motion
 AREA_COUNTERS_KIB = {
-    "ALEVB": (5809.15625, 168437.71875),
+    "ALEVB": (5783.09375, 168472.234375),
 }
 
 
