@@ -1451,6 +1451,21 @@ __device__ __forceinline__ double tile_row_fma(const typename Vec4<S>::type &m, 
 }
 
 // out_I = M_I v_I for every tile row (M = L^-1 with v = P b, or U^-1 with v = y)
+// out_I = Inv_I sv (sv: the tile row's 64 values in LDS), four threads per
+// output row, each summing 16 products in order before two shuffles
+template <class S>
+__device__ __forceinline__ void diag_mv(int64_t I, const S *__restrict__ inv, const double *sv,
+                                        double *__restrict__ out) {
+  const int i = threadIdx.x / 4, q = threadIdx.x % 4;
+  const S *m = inv + I * (int64_t)T * T + i * T + q * 16;
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) s = fma((double)m[k], sv[q * 16 + k], s);
+  s += __shfl_xor(s, 1, 64);
+  s += __shfl_xor(s, 2, 64);
+  if (q == 0) out[I * T + i] = s;
+}
+
 template <class S>
 __device__ __forceinline__ void diag_apply_body(int64_t bid, const S *__restrict__ inv,
                                                     const int32_t *__restrict__ piv,
@@ -1468,14 +1483,7 @@ __device__ __forceinline__ void diag_apply_body(int64_t bid, const S *__restrict
                                 : v[k];
   }
   __syncthreads();
-  const int i = threadIdx.x / 4, q = threadIdx.x % 4;
-  const S *m = inv + I * (int64_t)T * T + i * T + q * 16;
-  double s = 0.0;
-#pragma unroll
-  for (int k = 0; k < 16; ++k) s = fma((double)m[k], sv[q * 16 + k], s);
-  s += __shfl_xor(s, 1, 64);
-  s += __shfl_xor(s, 2, 64);
-  if (q == 0) out[I * T + i] = s;
+  diag_mv<S>(I, inv, sv, out);
 }
 
 template <class S>
