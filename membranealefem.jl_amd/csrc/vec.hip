@@ -9,22 +9,29 @@
 namespace mafem {
 
 
+// y = K x over the node blocks (ndf <= 8): eight lanes per row, lane c takes
+// column c of every block of the row's node (a wavefront reads whole blocks
+// coalesced), then the eight partial sums are added in a fixed tree
 __global__ __launch_bounds__(256) void k_spmv(int64_t nx, int ndf, const int32_t *__restrict__ rowptr,
                                               const int32_t *__restrict__ colidx,
                                               const double *__restrict__ bsr,
                                               const double *__restrict__ x,
                                               double *__restrict__ y) {
-  const int64_t R = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (R >= nx) return;
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t R = t >> 3;
+  const int c = (int)(t & 7);
+  if (R >= nx) return;  // whole groups of eight
   const int64_t p = R / ndf;
   const int r = (int)(R % ndf), bb = ndf * ndf;
   double s = 0.0;
-  for (int32_t k = rowptr[p]; k < rowptr[p + 1]; ++k) {
-    const double *blk = bsr + (int64_t)k * bb + r * ndf;
-    const double *xv = x + (int64_t)colidx[k] * ndf;
-    for (int c = 0; c < ndf; ++c) s = fma(blk[c], xv[c], s);
+  if (c < ndf) {
+    for (int32_t k = rowptr[p]; k < rowptr[p + 1]; ++k)
+      s = fma(bsr[(int64_t)k * bb + r * ndf + c], x[(int64_t)colidx[k] * ndf + c], s);
   }
-  y[R] = s;
+  s += __shfl_xor(s, 1, 8);
+  s += __shfl_xor(s, 2, 8);
+  s += __shfl_xor(s, 4, 8);
+  if (c == 0) y[R] = s;
 }
 
 __global__ __launch_bounds__(256) void k_axpby(int64_t n, double a, const double *__restrict__ x,
@@ -96,7 +103,7 @@ __global__ __launch_bounds__(256) void k_update_state(int64_t numnp, int ndf, in
 static inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 void bsr_spmv(Ctx &c, const double *d_x, double *d_y) {
-  hipLaunchKernelGGL(k_spmv, dim3(nblk(c.nx)), dim3(256), 0, c.stream, c.nx, c.ndf, c.d_rowptr,
+  hipLaunchKernelGGL(k_spmv, dim3(nblk(8 * c.nx)), dim3(256), 0, c.stream, c.nx, c.ndf, c.d_rowptr,
                      c.d_colidx, c.d_bsr, d_x, d_y);
   MAFEM_HIP(hipGetLastError());
 }
