@@ -59,12 +59,12 @@ def pull_speed(k, total=200, final=2.0):
 # hold the runs.
 STEP_COUNTERS_KIB = {
     "k_step_la": (16013.467625473484, 19804.575757575756),
-    "knd_step_la": (3673.986213235294, 3150.5367647058824),
-    "knd_schur": (157267.107421875, 93548.0),
+    "knd_step_la": (3675.1130514705883, 3150.5367647058824),
+    "knd_schur": (157148.615234375, 93548.0),
 }
 # the same counters per dispatch of the area kernel of the bench patch, by motion
 AREA_COUNTERS_KIB = {
-    "ALEVB": (5826.046875, 168434.21875),
+    "ALEVB": (5800.09375, 168459.0),
 }
 
 
