@@ -158,3 +158,27 @@ def test_nd_benchmark_patch():
     assert rel <= 1e-12 and relb <= 1e-12
     assert np.linalg.norm(K @ du + r) / np.linalg.norm(r) <= 1e-11
     assert np.linalg.norm(du - db) / np.linalg.norm(db) <= 1e-8
+
+
+@pytest.mark.parametrize("la,inv", [("1", "1"), ("1", "100000"), ("0", "1"), ("3", "16")])
+def test_nd_panel_forms_on_every_level(monkeypatch, la, inv):
+    """The panels as products with inverses formed by the LUs (levels of at
+    least MAFEM_ND_INV_FRONTS fronts) and the staged panels (the others), the
+    look-ahead steps on the top MAFEM_ND_LA_FRONTS-front levels: every mix
+    solves the 20x20 ALEVB tangent like SuperLU, its factors alone to 1e-9."""
+    monkeypatch.setenv("MAFEM_ND_LA_FRONTS", la)
+    monkeypatch.setenv("MAFEM_ND_INV_FRONTS", inv)
+    p, mesh, xms, cps = _setup(mafem.ALEVB, 20)
+    dev = mafem.analysis.DeviceMesh(mesh, p)
+    try:
+        dev.set_state(xms, cps)
+        r, K = dev.assemble(1e-3, 1e-3)
+        dev.set_ordering("nd")
+        dev.set_solver("direct", refine_sweeps=0, rtol=1e-13)
+        du, rel = dev.solve(-r)
+    finally:
+        dev.close()
+    print(f"nd la={la} inv={inv}: unrefined relres {rel:.2e}", flush=True)
+    assert rel <= 1e-9
+    ref = spla.spsolve(K.tocsc(), -r)
+    assert np.linalg.norm(du - ref) / np.linalg.norm(ref) <= 1e-7
