@@ -542,10 +542,13 @@ float elapsed(Ctx &c, int a, int b) {
 
 // Solves K x = b (extended vectors) with the factored band and FP64
 // iterative refinement against the block-sparse K; returns |b - K x| / |b|.
+// Refinement stops at tol, after max_sweeps, or once a sweep no longer halves
+// the residual (it has reached the rounding floor of K x for this b, as
+// LAPACK's and UMFPACK's refinement stop).
 double solve_refined(Ctx &c, const double *d_b, double *d_x, int max_sweeps, double tol) {
   const double bn = vec_norm2_masked(c, d_b);
   band_solve(c, d_b, d_x);
-  double rel = 0.0;
+  double rel = 0.0, prev = 0.0;
   int sweeps = 0;
   if (bn == 0.0) {
     c.stats.last_relres = 0.0;
@@ -556,7 +559,8 @@ double solve_refined(Ctx &c, const double *d_b, double *d_x, int max_sweeps, dou
     bsr_spmv(c, d_x, c.d_w);
     vec_axpby(c, c.nx, 1.0, d_b, -1.0, c.d_w);  // w = b - K x
     rel = vec_norm2_masked(c, c.d_w) / bn;
-    if (rel < tol || sweeps >= max_sweeps) break;
+    if (rel < tol || sweeps >= max_sweeps || (sweeps > 0 && rel > 0.5 * prev)) break;
+    prev = rel;
     band_solve(c, c.d_w, c.d_t);
     vec_axpby(c, c.nx, 1.0, c.d_t, 1.0, d_x);
     ++sweeps;
